@@ -1,0 +1,167 @@
+/*
+ * common.hpp -- shared definitions of the gfx950 SPH library: error handling, the
+ * smoothing kernel chosen at compile time, key constants and wavefront helpers.
+ *
+ * The smoothing kernel is fixed at build time as in the reference (include/kernel.hpp:
+ * 80-100, KERNEL=cubic|quintic): the build emits libcosmoSPHere.so (M4 cubic) and
+ * libcosmoSPHere_quintic.so (M6 quintic, -DCOSMO_KERNEL_QUINTIC) from the same sources.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+
+// Abort on any runtime error, naming the call site (reference convention,
+// include/gpu_check.hpp:14-22).
+#define HIP_CHECK(call)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            std::fprintf(stderr, "HIP error %s:%d  %s\n", __FILE__, __LINE__,            \
+                         hipGetErrorString(e_));                                         \
+            std::abort();                                                                \
+        }                                                                                \
+    } while (0)
+
+#define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
+namespace csph
+{
+
+constexpr int      kMaxLevel  = 21;                       // 63-bit Hilbert keys
+constexpr uint64_t kRootRange = uint64_t(1) << 63;        // 8^21
+constexpr uint64_t kDeadKey   = ~uint64_t(0);
+constexpr int      kWave      = 64;                       // CDNA wavefront
+constexpr int      kBucket    = 64;                       // leaf bucket size (tree parity size)
+constexpr double   kPi        = 3.141592653589793;
+
+#ifdef COSMO_KERNEL_QUINTIC
+constexpr double kRad    = 3.0;
+constexpr double kRad2   = 9.0;
+constexpr double kCnormk = 1.0 / (120.0 * kPi);
+constexpr double kHfactDefault = 1.0;
+constexpr const char* kKernelName = "M_6 quintic";
+#else
+constexpr double kRad    = 2.0;
+constexpr double kRad2   = 4.0;
+constexpr double kCnormk = 1.0 / kPi;
+constexpr double kHfactDefault = 1.2;
+constexpr const char* kKernelName = "M_4 cubic";
+#endif
+
+// M4 cubic (include/kernel.hpp:35-47) and M6 quintic (:53-78): W and dW/dq up to the
+// normalisation, in the arithmetic type T (double for the plain path, float for mixed).
+template<class T>
+__host__ __device__ inline void kernWdW(T q2, T q, T& w, T& dw)
+{
+#ifdef COSMO_KERNEL_QUINTIC
+    if (q < T(1))
+    {
+        const T q4 = q2 * q2;
+        w  = T(-10) * q4 * q + T(30) * q4 - T(60) * q2 + T(66);
+        dw = q * (T(-50) * q2 * q + T(120) * q2 - T(120));
+    }
+    else if (q < T(2))
+    {
+        const T a = q - T(3), b = q - T(2), a2 = a * a, b2 = b * b;
+        w  = -(a2 * a2 * a) + T(6) * (b2 * b2 * b);
+        dw = T(-5) * (a2 * a2) + T(30) * (b2 * b2);
+    }
+    else if (q < T(3))
+    {
+        const T a = q - T(3), a2 = a * a;
+        w  = -(a2 * a2 * a);
+        dw = T(-5) * (a2 * a2);
+    }
+    else { w = T(0); dw = T(0); }
+#else
+    (void)q2;
+    const T a = (T(2) - q > T(0)) ? (T(2) - q) : T(0);
+    const T b = (T(1) - q > T(0)) ? (T(1) - q) : T(0);
+    const T a2 = a * a, a3 = a2 * a, b2 = b * b, b3 = b2 * b;
+    w  = T(0.25) * (a3 - T(4) * b3);
+    dw = T(0.25) * (T(-3) * a2 + T(12) * b2);
+#endif
+}
+
+// Root box of one particle set (Cornerstone Box restated: lx = xmax - xmin, ilx = 1/lx).
+struct DBox
+{
+    double lo[3];
+    double hi[3];
+    double l[3];
+    double il[3];
+    int    periodic;
+    int    pad;
+};
+
+__host__ __device__ inline DBox makeDBox(const double b[6], int periodic)
+{
+    DBox box;
+    for (int a = 0; a < 3; ++a)
+    {
+        box.lo[a] = b[2 * a];
+        box.hi[a] = b[2 * a + 1];
+        box.l[a]  = box.hi[a] - box.lo[a];
+        box.il[a] = 1.0 / box.l[a];
+    }
+    box.periodic = periodic;
+    box.pad      = 0;
+    return box;
+}
+
+// ---------------------------------------------------------------------------------------
+// Wavefront helpers (wave64).
+// ---------------------------------------------------------------------------------------
+__device__ inline int laneId() { return __lane_id(); }
+
+__device__ inline uint64_t ballot64(bool p) { return __ballot(p); }
+
+__device__ inline uint64_t lanemaskLt() { return (uint64_t(1) << laneId()) - 1; }
+
+template<class T>
+__device__ inline T waveReduceSum(T v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template<class T>
+__device__ inline T waveReduceMax(T v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const T o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Inclusive prefix sum over the wave.
+template<class T>
+__device__ inline T waveInclusiveScan(T v)
+{
+    const int lane = laneId();
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const T o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+// Orders memory operations of one wavefront through LDS: what a lane stored is visible
+// to the other lanes of the same wave after this point.
+__device__ inline void waveLdsSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+inline int iceil(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+} // namespace csph

@@ -1,0 +1,296 @@
+/*
+ * jlist.hip -- smoothing-length maxima and the neighbour-leaf (j-leaf) lists.
+ *
+ * Restates include/tree.cuh:176-389 and src/force.cu:19-88.  The list of an i-leaf is the
+ * reference's depth-first walk: test the root; then for each popped node visit its 8
+ * children in key order, append accepted leaves, push accepted internal nodes, pop LIFO.
+ *
+ * Mapping for gfx950: a group of 8 lanes walks one i-leaf, lane o testing child o of the
+ * current node, so the 8 children are tested at once and read with unit stride from the
+ * SoA node geometry; a ballot per group turns the accepted children into list slots (in
+ * octant order) and stack pushes (in octant order, so pops are LIFO exactly as the
+ * reference's).  Eight groups share a wavefront (64 lanes busy), and each group's stack
+ * lives in LDS instead of a 768-byte private array.  A LIFO depth-first stack never holds
+ * more than 7 siblings per level plus the 8 children just pushed: 7*20 + 8 = 148 entries.
+ *
+ * The accept tests are evaluated exactly as written (no contraction), which keeps the
+ * lists bit-identical to the FP64 oracle's.
+ */
+#include "tree.hpp"
+
+#include <vector>
+
+namespace csph
+{
+
+// hmax_leaf = max finite h of the leaf's particles: include/tree.cuh:177-191.
+// One wavefront per leaf, 4 per block.
+__global__ __launch_bounds__(256) void hmaxLeafKernel(const double* __restrict__ h, const uint32_t* __restrict__ layout,
+                                                      const int* __restrict__ leaves, int nLeaves,
+                                                      double* __restrict__ hmaxLeaf)
+{
+    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= nLeaves) return;
+    const int l = leaves ? leaves[a] : a;
+    const uint32_t p0 = layout[l], p1 = layout[l + 1];
+    double m = 0.0;
+    for (uint32_t p = p0 + laneId(); p < p1; p += kWave)
+    {
+        const double v = h[p];
+        if (isfinite(v)) m = fmax(m, v);
+    }
+    m = waveReduceMax(m);
+    if (laneId() == 0) hmaxLeaf[l] = m;
+}
+
+// Upsweep: include/tree.cuh:196-217 (a leaf node takes its hmax_leaf if finite and > 0,
+// else 0; an internal node the max of its 8 children).
+__global__ void nodeHmaxLeafKernel(const double* __restrict__ hmaxLeaf, const int* __restrict__ leafNode, int nLeaves,
+                                   double* __restrict__ nodeHmax)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nLeaves) return;
+    const double hm = hmaxLeaf[l];
+    nodeHmax[leafNode[l]] = (isfinite(hm) && hm > 0.0) ? hm : 0.0;
+}
+
+__global__ void nodeHmaxInternalKernel(const int* __restrict__ intLevel, const int* __restrict__ intNode, int nInt,
+                                       int level, const int* __restrict__ nodeChild, double* __restrict__ nodeHmax)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nInt || intLevel[k] != level) return;
+    const int node = intNode[k];
+    const int c = nodeChild[node];
+    double m = 0.0;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) m = fmax(m, nodeHmax[c + o]);
+    nodeHmax[node] = m;
+}
+
+// Cornerstone minDistance (box-box), squared: |cb - ca| - sa - sb, clipped at 0 as
+// (d + |d|) / 2; periodic: nearest image of cb - ca first.
+template<bool PER>
+__device__ inline double boxDist2(const double ac[3], const double as[3], double bcx, double bcy, double bcz, double bsx,
+                                  double bsy, double bsz, const DBox& box)
+{
+#pragma clang fp contract(off)
+    const double bc[3] = {bcx, bcy, bcz}, bs[3] = {bsx, bsy, bsz};
+    double d[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        double dx = bc[a] - ac[a];
+        if constexpr (PER) dx -= box.l[a] * rint(dx * box.il[a]);
+        double v = fabs(dx) - as[a] - bs[a];
+        v += fabs(v);
+        v *= 0.5;
+        d[a] = v;
+    }
+    return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+constexpr int kListThreads = 256;
+constexpr int kGroups      = kListThreads / 8;   // i-leaves per block
+constexpr int kStack       = 152;                // >= 7*20 + 8
+
+struct ListArgs
+{
+    const int*      leaves;
+    int             nLeaves;
+    const int*      leafNode;
+    const int*      leafSet;
+    const int*      nodeFirst;
+    const int*      nodeChild;
+    const double*   cx; const double* cy; const double* cz;
+    const double*   sx; const double* sy; const double* sz;
+    const double*   hmaxLeaf;
+    const double*   nodeHmax;
+    const DBox*     boxes;
+    const uint32_t* jOffset;
+    int*            jList;
+    uint32_t*       jCount;
+    int*            ovf;
+    int             nNodes;   // walk cap: a tree walk visits each node at most once
+};
+
+template<bool SYM, bool COUNT, bool PER>
+__global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
+{
+#pragma clang fp contract(off)
+    __shared__ int stk[kGroups][kStack];
+    const int g = threadIdx.x >> 3, o = threadIdx.x & 7;
+    const int a = blockIdx.x * kGroups + g;
+    if (a >= A.nLeaves) return;
+    const int shift = (laneId() >> 3) << 3;
+    const uint32_t below = (1u << o) - 1u;
+
+    const int iLeaf = A.leaves ? A.leaves[a] : a;
+    const int in = A.leafNode[iLeaf];
+    const int b = A.leafSet[iLeaf];
+    const DBox box = A.boxes[b];
+    const double ic[3] = {A.cx[in], A.cy[in], A.cz[in]};
+    const double is[3] = {A.sx[in], A.sy[in], A.sz[in]};
+    double hmax = A.hmaxLeaf[iLeaf];
+    if (!isfinite(hmax) || hmax <= 0.0) hmax = 0.0;
+    const double radHi = kRad * hmax;
+    const double aSize[3] = {SYM ? is[0] : is[0] + radHi, SYM ? is[1] : is[1] + radHi, SYM ? is[2] : is[2] + radHi};
+
+    auto inRange = [&](int node) -> bool {
+        const double d2 = boxDist2<PER>(ic, aSize, A.cx[node], A.cy[node], A.cz[node], A.sx[node], A.sy[node],
+                                        A.sz[node], box);
+        if constexpr (SYM)
+        {
+            const double rs = fmax(radHi, kRad * A.nodeHmax[node]);
+            return d2 <= rs * rs;
+        }
+        else
+        {
+            return d2 <= 0.0;
+        }
+    };
+
+    const uint32_t base = COUNT ? 0u : A.jOffset[a];
+    const int root = A.nodeFirst[b];
+    if (!inRange(root))
+    {
+        if (COUNT && o == 0) A.jCount[a] = 0;
+        return;
+    }
+    const int rc = A.nodeChild[root];
+    if (rc < 0)
+    {
+        if (o == 0)
+        {
+            if (COUNT) A.jCount[a] = 1;
+            else       A.jList[base] = ~rc;
+        }
+        return;
+    }
+
+    const uint32_t limit = COUNT ? 0u : A.jOffset[a + 1];
+    uint32_t count = 0;
+    int sp = 0, node = root, steps = 0;
+    while (true)
+    {
+        if (++steps > A.nNodes)
+        {
+            if (o == 0) atomicAdd(&A.ovf[1], 1);
+            break;
+        }
+        const int  c    = A.nodeChild[node] + o;
+        const bool acc  = inRange(c);
+        const int  cref = A.nodeChild[c];
+        const bool isLeaf = acc && cref < 0;
+        const bool isInt  = acc && cref >= 0;
+        const uint32_t mL = (uint32_t)(ballot64(isLeaf) >> shift) & 0xFFu;
+        const uint32_t mI = (uint32_t)(ballot64(isInt) >> shift) & 0xFFu;
+        if (!COUNT && isLeaf)
+        {
+            const uint32_t at = base + count + __popc(mL & below);
+            if (at < limit) A.jList[at] = ~cref;
+            else            atomicAdd(&A.ovf[0], 1);
+        }
+        count += __popc(mL);
+        const int push = __popc(mI);
+        if (sp + push <= kStack)
+        {
+            if (isInt) stk[g][sp + __popc(mI & below)] = c;
+            sp += push;
+        }
+        else if (o == 0)
+        {
+            atomicAdd(&A.ovf[1], 1);   // subtree dropped; cannot happen for a depth-21 tree
+        }
+        waveLdsSync();
+        if (sp == 0) break;
+        node = stk[g][--sp];
+        waveLdsSync();
+    }
+    if (COUNT && o == 0) A.jCount[a] = count;
+}
+
+// ---------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------
+void hmaxLeaves(State& s, const int* leaves, int nLeaves)
+{
+    if (nLeaves <= 0) return;
+    double* hm = s.hmaxLeaf.get<double>(s.nLeaves + 1);
+    hmaxLeafKernel<<<iceil(nLeaves, 4), 256, 0, s.stream>>>(s.h.as<double>(), s.layout.as<uint32_t>(), leaves, nLeaves,
+                                                             hm);
+    LAUNCH_CHECK();
+}
+
+void hmaxUpsweep(State& s)
+{
+    double* nh = s.nodeHmax.get<double>(s.nNodes + 1);
+    nodeHmaxLeafKernel<<<iceil(s.nLeaves, 256), 256, 0, s.stream>>>(s.hmaxLeaf.as<double>(), s.leafNode.as<int>(),
+                                                                      s.nLeaves, nh);
+    LAUNCH_CHECK();
+    for (int lvl = kMaxLevel - 1; lvl >= 0; --lvl)
+    {
+        if (s.nInternal == 0) break;
+        nodeHmaxInternalKernel<<<iceil(s.nInternal, 256), 256, 0, s.stream>>>(
+            s.intLevel.as<int>(), s.intNode.as<int>(), s.nInternal, lvl, s.nodeChild.as<int>(), nh);
+        LAUNCH_CHECK();
+    }
+}
+
+template<bool SYM, bool COUNT>
+static void launchList(const ListArgs& a, bool periodic, hipStream_t st)
+{
+    const int grid = iceil(a.nLeaves, kGroups);
+    if (periodic) jlistKernel<SYM, COUNT, true><<<grid, kListThreads, 0, st>>>(a);
+    else          jlistKernel<SYM, COUNT, false><<<grid, kListThreads, 0, st>>>(a);
+    LAUNCH_CHECK();
+}
+
+long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric)
+{
+    hipStream_t st = s.stream;
+    uint32_t* jc = s.jCount.get<uint32_t>(nLeaves + 1);
+    uint32_t* jo = s.jOffset.get<uint32_t>(nLeaves + 2);
+    ListArgs a{};
+    a.leaves = leaves;
+    a.nLeaves = nLeaves;
+    a.leafNode = s.leafNode.as<int>();
+    a.leafSet = s.leafSet.as<int>();
+    a.nodeFirst = s.intFirst.as<int>() + 3 * s.nSets;
+    a.nodeChild = s.nodeChild.as<int>();
+    a.cx = s.nodeCx.as<double>(); a.cy = s.nodeCy.as<double>(); a.cz = s.nodeCz.as<double>();
+    a.sx = s.nodeSx.as<double>(); a.sy = s.nodeSy.as<double>(); a.sz = s.nodeSz.as<double>();
+    a.hmaxLeaf = s.hmaxLeaf.as<double>();
+    a.nodeHmax = symmetric ? s.nodeHmax.as<double>() : nullptr;
+    a.boxes = s.box.as<DBox>();
+    a.jCount = jc;
+    a.ovf = s.ovf.get<int>(4);
+    a.nNodes = s.nNodes;
+    const bool per = s.hBox.size() && s.hBox[0].periodic;
+    if (nLeaves > 0)
+    {
+        if (symmetric) launchList<true, true>(a, per, st);
+        else           launchList<false, true>(a, per, st);
+    }
+    unsigned long long* tot = s.counters.get<unsigned long long>(64) + 32;
+    fillU32(reinterpret_cast<uint32_t*>(tot), 0u, 2, st);
+    scanExclusive<uint32_t>(jc, jo, nLeaves, s.scanTmp, tot, st);
+    unsigned long long total = 0;
+    HIP_CHECK(hipMemcpyAsync(&total, tot, sizeof(total), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (total > 0xFFFFFFF0ull)
+    {
+        std::fprintf(stderr, "FATAL: neighbour-leaf lists hold %llu entries, beyond 32-bit offsets\n", total);
+        std::abort();
+    }
+    a.jOffset = jo;
+    a.jList = s.jList.get<int>(total + 1);
+    if (nLeaves > 0)
+    {
+        if (symmetric) launchList<true, false>(a, per, st);
+        else           launchList<false, false>(a, per, st);
+    }
+    s.listTotal = (int)total;
+    return (long long)total;
+}
+
+} // namespace csph

@@ -1,0 +1,341 @@
+/*
+ * solver.hip -- host drivers: the density / smoothing-length solve and the force pass.
+ *
+ * solveDensH restates src/density_base.cu:720-1015: build the tree, iterate Newton-Raphson
+ * on the shrinking active set (hmax of the active leaves -> their neighbour-leaf lists ->
+ * one density kernel -> compaction), abort on an unconverged solve, check the periodic
+ * support, optionally sweep the gradients at the converged h, and bump the token the force
+ * pass checks.  computeForces restates src/force.cu:385-455.
+ *
+ * Differences of form, not of result: lists are built for the active leaves only (the
+ * reference builds all leaves and reads only the active ones; tree.cuh:455-491), and the
+ * active set is kept as a per-particle flag plus a compacted leaf list rather than
+ * copy_if + sort + unique (density_base.cu:856-886).  The host reads two scalars per
+ * iteration (list total, active-leaf count).
+ */
+#include "density.hpp"
+#include "solver.hpp"
+#include "tree.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace csph
+{
+
+State& state()
+{
+    static State* s = new State();   // deliberately leaked, see state.hpp
+    return *s;
+}
+
+// The stream is created on the first call that does device work, so that argument checks
+// (which abort before any device call, as the reference's) need no device at all.
+void ensureStream(State& s)
+{
+    if (!s.stream) HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+}
+
+static SortScratch& sortScratch()
+{
+    static SortScratch* sc = new SortScratch();
+    return *sc;
+}
+
+void sizeParticleArrays(State& s, int n)
+{
+    if (s.sizedFor == n) return;
+    ensureStream(s);
+    const size_t m = (size_t)std::max(n, 1) + 1;
+    for (DevBuf* b : {&s.x, &s.y, &s.z, &s.h, &s.vx, &s.vy, &s.vz, &s.ax, &s.ay, &s.az, &s.rho, &s.gradh, &s.divv, &s.xi,
+                      &s.ddivvdt, &s.pro2, &s.cs, &s.alpha, &s.u, &s.fx, &s.fy, &s.fz, &s.f4, &s.vsig, &s.divvF, &s.stage,
+                      &s.hsqinv, &s.hinv, &s.rhoh, &s.rho1, &s.grkfac, &s.pres, &s.auterm, &s.divfac})
+        b->get<double>(m);
+    s.order.get<int>(m);
+    s.particleLeaf.get<int>(m);
+    s.active.get<uint8_t>(m);
+    s.sizedFor = n;
+}
+
+struct Ev
+{
+    hipEvent_t e;
+    Ev() { HIP_CHECK(hipEventCreate(&e)); }
+    ~Ev() { (void)hipEventDestroy(e); }
+    void rec(hipStream_t st) { HIP_CHECK(hipEventRecord(e, st)); }
+};
+
+static double secs(const Ev& a, const Ev& b)
+{
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, a.e, b.e));
+    return ms * 1e-3;
+}
+
+static unsigned long long* zeroCounters(State& s)
+{
+    unsigned long long* c = s.counters.get<unsigned long long>((size_t)counterSlots() * counterStride() + 64);
+    fillU32(reinterpret_cast<uint32_t*>(c), 0u, 2 * (size_t)counterSlots() * counterStride(), s.stream);
+    return c;
+}
+
+// Sums counter k over the slots (after the stream is synchronised).
+static void readCounters(State& s, unsigned long long out[3])
+{
+    const int ns = counterSlots(), st = counterStride();
+    std::vector<unsigned long long> h((size_t)ns * st);
+    HIP_CHECK(hipMemcpyAsync(h.data(), s.counters.as<unsigned long long>(), h.size() * 8, hipMemcpyDeviceToHost,
+                             s.stream));
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    out[0] = out[1] = out[2] = 0;
+    for (int k = 0; k < ns; ++k)
+        for (int c = 0; c < 3; ++c) out[c] += h[(size_t)k * st + c];
+}
+
+static bool periodicOf(const State& s) { return !s.hBox.empty() && s.hBox[0].periodic; }
+
+static DensArgs densArgs(State& s, const int* leaves, int nLeaves, double pmass, double tolh, double hfact,
+                         unsigned long long* counters)
+{
+    DensArgs a{};
+    a.leaves = leaves;
+    a.nLeaves = nLeaves;
+    a.jOffset = s.jOffset.as<uint32_t>();
+    a.jList = s.jList.as<int>();
+    a.layout = s.layout.as<uint32_t>();
+    a.leafNode = s.leafNode.as<int>();
+    a.leafSet = s.leafSet.as<int>();
+    a.cx = s.nodeCx.as<double>(); a.cy = s.nodeCy.as<double>(); a.cz = s.nodeCz.as<double>();
+    a.x = s.x.as<double>(); a.y = s.y.as<double>(); a.z = s.z.as<double>();
+    a.h = s.h.as<double>();
+    a.rho = s.rho.as<double>();
+    a.gradh = s.gradh.as<double>();
+    a.active = s.active.as<uint8_t>();
+    a.boxes = s.box.as<DBox>();
+    a.pmass = pmass;
+    a.tolh = tolh;
+    a.hfact = hfact;
+    a.counters = counters;
+    return a;
+}
+
+Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const double* periodicBox, double tolh,
+                   double hfact)
+{
+    hipStream_t st = s.stream;
+    Timings t{};
+    if (!(hfact > 0.0)) hfact = kHfactDefault;
+    s.hfact = hfact;
+    s.lastPmass = pmass;
+    const bool mixed = s.precision == kMixed;
+    const bool closure = (s.closureOn < 0) ? mixed : (s.closureOn != 0);
+
+    Ev e0, e1;
+    e0.rec(st);
+    int* ovf = s.ovf.get<int>(4);
+    fillU32(reinterpret_cast<uint32_t*>(ovf), 0u, 4, st);
+    s.t = Timings{};
+    buildTree(s, withGradients, periodicBox, sortScratch());
+    t = s.t;
+    const int n = s.n, na = s.totalAlive, L = s.nLeaves;
+    const bool per = periodicOf(s);
+
+    launchInitActive(s.active.as<uint8_t>(), na, n, s.rho.as<double>(), s.gradh.as<double>(), st);
+    int* leaves = s.activeLeaves.get<int>(L + 1);
+    int* leavesAlt = s.activeLeavesAlt.get<int>(L + 1);
+    uint32_t* flag = s.leafFlag.get<uint32_t>(2 * (size_t)L + 4);
+    uint32_t* pos = flag + L + 2;
+    launchIota(leaves, L, st);
+    int nAL = na > 0 ? L : 0;
+    unsigned long long nActive = (unsigned long long)na;
+
+    Ev a0, a1, a2, a3;
+    for (int iter = 0; iter < s.maxIter && nAL > 0; ++iter)
+    {
+        a0.rec(st);
+        hmaxLeaves(s, leaves, nAL);
+        buildLists(s, leaves, nAL, false);
+        a1.rec(st);
+        unsigned long long* ctr = zeroCounters(s);
+        launchDensity(densArgs(s, leaves, nAL, pmass, tolh, hfact, ctr), s.precision, mode, per, st);
+        a2.rec(st);
+        launchLeafFlags(leaves, nAL, s.layout.as<uint32_t>(), s.active.as<uint8_t>(), flag, ctr + 2, st);
+        scanExclusive<uint32_t>(flag, pos, nAL, s.scanTmp, nullptr, st);
+        launchLeafScatter(leaves, nAL, flag, pos, leavesAlt, st);
+        a3.rec(st);
+        unsigned int nNext = 0;
+        HIP_CHECK(hipMemcpyAsync(&nNext, pos + nAL, sizeof(nNext), hipMemcpyDeviceToHost, st));
+        unsigned long long c[3];
+        readCounters(s, c);   // synchronises
+        t.jbuild += secs(a0, a1);
+        t.nrkern += secs(a1, a2);
+        t.compact += secs(a2, a3);
+        t.pairsAccepted += (double)c[0];
+        t.pairsCandidate += (double)c[1];
+        t.kernLaunches += 1;
+        nActive = c[2];
+        t.iters = iter + 1;
+        std::swap(leaves, leavesAlt);
+        std::swap(s.activeLeaves, s.activeLeavesAlt);
+        nAL = (int)nNext;
+    }
+    if (na == 0) nActive = 0;
+
+    int hovf[2] = {0, 0};
+    HIP_CHECK(hipMemcpyAsync(hovf, ovf, sizeof(hovf), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (nActive > 0 || hovf[0] > 0 || hovf[1] > 0)
+    {
+        std::fprintf(stderr, "FATAL: solveDensH: unconverged=%llu jlist_trunc=%d stack_drops=%d (iters=%d)\n",
+                     nActive, hovf[0], hovf[1], t.iters);
+        std::abort();
+    }
+    if (per && na > 0)
+    {
+        unsigned long long* m = s.counters.as<unsigned long long>() + (size_t)counterSlots() * counterStride() + 8;
+        fillU32(reinterpret_cast<uint32_t*>(m), 0u, 2, st);
+        launchMaxH(s.h.as<double>(), na, m, st);
+        unsigned long long hm = 0;
+        HIP_CHECK(hipMemcpyAsync(&hm, m, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        double hmax;
+        std::memcpy(&hmax, &hm, 8);
+        double lmin = INFINITY;
+        for (const DBox& b : s.hBox) lmin = std::min({lmin, b.l[0], b.l[1], b.l[2]});
+        if (kRad * hmax >= 0.5 * lmin)
+        {
+            std::fprintf(stderr,
+                         "FATAL: solveDensH: kernel support %gh = %g reaches half the periodic box "
+                         "(smallest side %g); too few particles for this box\n",
+                         kRad, kRad * hmax, lmin);
+            std::abort();
+        }
+    }
+
+    // closure (mixed density-only form): rho and d(rho)/d(h) at the converged h
+    if (closure && !withGradients && L > 0 && na > 0)
+    {
+        Ev c0, c1;
+        c0.rec(st);
+        hmaxLeaves(s, nullptr, L);
+        buildLists(s, nullptr, L, false);
+        unsigned long long* ctr = zeroCounters(s);
+        launchDensity(densArgs(s, nullptr, L, pmass, tolh, hfact, ctr), s.precision, 2, per, st);
+        c1.rec(st);
+        unsigned long long c[3];
+        readCounters(s, c);
+        t.closure = secs(c0, c1);
+        t.closurePairs = (double)c[0];
+        t.closureCand = (double)c[1];
+        s.listLeaves = L;
+        s.listSymmetric = 0;
+    }
+
+    if (withGradients)
+    {
+        Ev g0, g1, g2;
+        g0.rec(st);
+        hmaxLeaves(s, nullptr, L);
+        buildLists(s, nullptr, L, false);
+        g1.rec(st);
+        GradArgs g{};
+        g.nLeaves = na > 0 ? L : 0;
+        g.jOffset = s.jOffset.as<uint32_t>();
+        g.jList = s.jList.as<int>();
+        g.layout = s.layout.as<uint32_t>();
+        g.leafSet = s.leafSet.as<int>();
+        g.x = s.x.as<double>(); g.y = s.y.as<double>(); g.z = s.z.as<double>();
+        g.vx = s.vx.as<double>(); g.vy = s.vy.as<double>(); g.vz = s.vz.as<double>();
+        g.ax = s.ax.as<double>(); g.ay = s.ay.as<double>(); g.az = s.az.as<double>();
+        g.h = s.h.as<double>();
+        g.rho = s.rho.as<double>(); g.gradh = s.gradh.as<double>();
+        g.divv = s.divv.as<double>(); g.xi = s.xi.as<double>(); g.ddivvdt = s.ddivvdt.as<double>();
+        g.boxes = s.box.as<DBox>();
+        g.pmass = pmass;
+        launchGradients(g, per, st);
+        launchDeadGradients(na, n, s.divv.as<double>(), s.xi.as<double>(), s.ddivvdt.as<double>(), st);
+        g2.rec(st);
+        HIP_CHECK(hipEventSynchronize(g2.e));
+        t.gjbuild = secs(g0, g1);
+        t.gradkern = secs(g1, g2);
+        s.listLeaves = L;
+        s.listSymmetric = 0;
+    }
+    e1.rec(st);
+    HIP_CHECK(hipEventSynchronize(e1.e));
+    t.total = secs(e0, e1);
+    s.t = t;
+    ++s.token;
+    return t;
+}
+
+ForceTimings computeForces(State& s, double pmass, double beta, double alphau, bool disc, bool pdv, bool shock)
+{
+    hipStream_t st = s.stream;
+    ForceTimings ft{};
+    const int n = s.n, L = s.nLeaves, na = s.totalAlive;
+    const bool per = periodicOf(s);
+    Ev e0, e1, e2, e3;
+    e0.rec(st);
+    if (s.jlistToken != s.token || !s.listSymmetric)
+    {
+        int* ovf = s.ovf.get<int>(4);
+        fillU32(reinterpret_cast<uint32_t*>(ovf), 0u, 4, st);
+        hmaxLeaves(s, nullptr, L);
+        hmaxUpsweep(s);
+        buildLists(s, nullptr, L, true);
+        s.listLeaves = L;
+        s.listSymmetric = 1;
+        s.jlistToken = s.token;
+        int hovf[2] = {0, 0};
+        HIP_CHECK(hipMemcpyAsync(hovf, ovf, sizeof(hovf), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        if (hovf[0] > 0 || hovf[1] > 0)
+            std::fprintf(stderr, "WARNING! buildForceJLeafList: jlist_trunc=%d stack_drops=%d\n", hovf[0], hovf[1]);
+    }
+    e1.rec(st);
+    launchForcePrep(s.h.as<double>(), s.gradh.as<double>(), s.pro2.as<double>(), s.hsqinv.as<double>(),
+                    s.hinv.as<double>(), s.rhoh.as<double>(), s.rho1.as<double>(), s.grkfac.as<double>(),
+                    s.pres.as<double>(), s.auterm.as<double>(), s.divfac.as<double>(), n, pmass, alphau, s.hfact, st);
+    e2.rec(st);
+    unsigned long long* ctr = zeroCounters(s);
+    ForceArgs a{};
+    a.nLeaves = na > 0 ? L : 0;
+    a.jOffset = s.jOffset.as<uint32_t>();
+    a.jList = s.jList.as<int>();
+    a.layout = s.layout.as<uint32_t>();
+    a.leafSet = s.leafSet.as<int>();
+    a.x = s.x.as<double>(); a.y = s.y.as<double>(); a.z = s.z.as<double>();
+    a.vx = s.vx.as<double>(); a.vy = s.vy.as<double>(); a.vz = s.vz.as<double>();
+    a.h = s.h.as<double>();
+    a.pro2 = s.pro2.as<double>(); a.cs = s.cs.as<double>(); a.alpha = s.alpha.as<double>(); a.u = s.u.as<double>();
+    a.hsqinv = s.hsqinv.as<double>(); a.hinv = s.hinv.as<double>(); a.rhoh = s.rhoh.as<double>();
+    a.rho1 = s.rho1.as<double>(); a.grkfac = s.grkfac.as<double>(); a.pres = s.pres.as<double>();
+    a.auterm = s.auterm.as<double>(); a.divfac = s.divfac.as<double>();
+    a.fx = s.fx.as<double>(); a.fy = s.fy.as<double>(); a.fz = s.fz.as<double>();
+    a.f4 = s.f4.as<double>(); a.vsig = s.vsig.as<double>(); a.divv = s.divvF.as<double>();
+    a.boxes = s.box.as<DBox>();
+    a.pmass = pmass;
+    a.beta = beta;
+    a.pdv = pdv ? 1 : 0;
+    a.shock = shock ? 1 : 0;
+    a.counters = ctr;
+    launchForce(a, per, disc, pdv && shock, st);
+    launchDeadForce(na, n, s.fx.as<double>(), s.fy.as<double>(), s.fz.as<double>(), s.f4.as<double>(),
+                    s.vsig.as<double>(), s.divvF.as<double>(), st);
+    e3.rec(st);
+    unsigned long long c[3];
+    readCounters(s, c);
+    ft.upsweep = 0.0;
+    ft.jbuild = secs(e0, e1);
+    ft.prep = secs(e1, e2);
+    ft.kernel = secs(e2, e3);
+    ft.total = secs(e0, e3);
+    ft.pairsAccepted = (double)c[0];
+    ft.pairsCandidate = (double)c[1];
+    s.ft = ft;
+    return ft;
+}
+
+} // namespace csph

@@ -1,0 +1,149 @@
+/*
+ * state.hpp -- device state that outlives a call (the MI355X counterpart of the
+ * reference's GpuState, include/gpu_state.hpp:36-139).
+ *
+ * Layout in HBM: every particle field is its own FP64 array (SoA), in Hilbert order after
+ * a solve, so a leaf's particles are one contiguous run of each array and a wavefront
+ * reads them with unit stride.  Buffers grow and are never shrunk; nothing is freed
+ * during a solve.  The state is created on first use and deliberately never destroyed
+ * (the reference's leaked-global convention, src/gpu_state.cu:4-19).
+ *
+ * Particle sets: everything below is batched.  A single Phantom solve is a batch of one
+ * set; the ensemble API packs many independent sets into the same arrays.  After a solve
+ * the live particles of set b occupy [aliveFirst[b], aliveFirst[b] + nAlive[b]) of the
+ * sorted arrays, sets in order, and every dead particle (h <= 0) sits after all of them.
+ */
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace csph
+{
+
+// Grow-only device allocation.
+struct DevBuf
+{
+    void*  p   = nullptr;
+    size_t cap = 0;
+
+    void reserve(size_t bytes)
+    {
+        if (bytes <= cap) return;
+        if (p) HIP_CHECK(hipFree(p));
+        size_t want = bytes + bytes / 8 + 256;
+        HIP_CHECK(hipMalloc(&p, want));
+        cap = want;
+    }
+    template<class T> T* as() const { return static_cast<T*>(p); }
+    template<class T> T* get(size_t n) { reserve(n * sizeof(T)); return as<T>(); }
+};
+
+// Per-phase device time of one solve (seconds), and work counters.
+struct Timings
+{
+    double bbox = 0, keys = 0, sort = 0, permute = 0, tree = 0, nodes = 0;
+    double jbuild = 0, nrkern = 0, compact = 0, gjbuild = 0, gradkern = 0, closure = 0;
+    double total = 0;
+    int    iters = 0;
+    int    nLeaves = 0;
+    double pairsAccepted = 0, pairsCandidate = 0;   // summed over NR iterations
+    double kernLaunches = 0;                         // NR pair-kernel launches
+    double closurePairs = 0, closureCand = 0;
+};
+
+struct ForceTimings
+{
+    double upsweep = 0, jbuild = 0, prep = 0, kernel = 0, total = 0;
+    double pairsAccepted = 0, pairsCandidate = 0;
+};
+
+enum Precision : int { kPlainFP64 = 0, kMixed = 1 };
+enum Mapping : int { kMapAuto = 0, kMapWavePerLeaf = 1, kMapFlat = 2 };
+
+struct State
+{
+    hipStream_t stream = nullptr;
+
+    // ---- particle arrays (length capacity >= n) ----
+    int n = 0;          // particles of the last solve (sum over sets)
+    int sizedFor = 0;   // cosmo_arrays_init count, 0 if never
+    DevBuf x, y, z, h, vx, vy, vz, ax, ay, az;
+    DevBuf rho, gradh, divv, xi, ddivvdt;
+    DevBuf pro2, cs, alpha, u;
+    DevBuf fx, fy, fz, f4, vsig, divvF;
+    DevBuf order;       // sorted index -> caller index (int32)
+    int    haveOrder = 0;
+    DevBuf stage;       // transfer staging (n doubles)
+    DevBuf stash;       // pristine copy of POS|HSML|VEL|ACCEL for repeated benchmark solves
+    int    stashN = 0;
+
+    // ---- sets ----
+    int nSets = 0;
+    std::vector<int> hSetFirst, hSetN;        // caller-side ranges, input order
+    std::vector<int> hAliveFirst, hAlive;     // sorted ranges of live particles
+    std::vector<int> hLeafFirst, hLeafN, hIntFirst, hIntN, hNodeFirst;
+    std::vector<DBox> hBox;
+    DevBuf setFirst, setN, aliveFirst, alive, leafFirst, intFirst, nodeFirst, box;
+    int totalAlive = 0;
+
+    // ---- sort / tree scratch ----
+    DevBuf keys, keysAlt, vals, valsAlt, hist, sortSetKey;
+    DevBuf partials, boxDesc;
+    DevBuf permAlt[10];                       // permute targets, swapped with the fields
+    DevBuf tmpA, tmpB;                        // particle-sized scratch for the permute
+    DevBuf winCount, winOffset;               // internal-node emission
+    DevBuf preStart, preLevel, preSet;        // internal nodes in key (pre-)order
+    DevBuf bfsKey, bfsKeyAlt, bfsIdx, bfsIdxAlt;
+    DevBuf intStart, intLevel, intSet;        // internal nodes in (set, level, key) order
+    DevBuf scanTmp;
+
+    // ---- tree ----
+    int nLeaves = 0, nInternal = 0, nNodes = 0;
+    DevBuf leafStart;    // u64 start key of each leaf (the csTree without its end)
+    DevBuf leafLevel;    // int
+    DevBuf leafSet;      // int
+    DevBuf layout;       // u32, nLeaves + 1: leaf l owns sorted particles [layout[l], layout[l+1])
+    DevBuf leafNode;     // int: node id of each leaf
+    DevBuf particleLeaf; // int
+    DevBuf nodeChild;    // int: first child node (internal) or ~leaf (leaf)
+    DevBuf nodeCx, nodeCy, nodeCz, nodeSx, nodeSy, nodeSz;   // FP64 geometry (SoA)
+    DevBuf nodeHmax;     // force upsweep
+    DevBuf intNode;      // node id of each internal node (BFS order)
+    std::vector<int> hIntLevelFirst;          // level ranges of internal nodes (all sets)
+
+    // ---- NR and lists ----
+    DevBuf hmaxLeaf;
+    DevBuf activeLeaves, activeLeavesAlt, leafFlag;
+    DevBuf active;       // u8 per particle: still iterating
+    DevBuf converged;    // u8 per particle
+    DevBuf jCount, jOffset, jList;
+    int    listLeaves = 0;    // number of leaves the CSR lists cover (all-leaf lists)
+    int    listTotal  = 0;
+    int    listSymmetric = 0; // 1 when the lists are the force (symmetric) lists
+    DevBuf counters;          // u64 device counters (pairs, candidates, unconverged)
+    DevBuf pinnedCounters;    // host-visible copy target
+    DevBuf ovf;               // int[2] list truncation / stack drop (never expected)
+
+    // ---- force prep ----
+    DevBuf hsqinv, hinv, rhoh, rho1, grkfac, pres, auterm, divfac;
+
+    // ---- bookkeeping ----
+    double   hfact = 0.0;
+    double   lastPmass = 0.0;
+    uint64_t token = 0, jlistToken = 0;
+    int      maxIter = 100;
+    int      precision = kPlainFP64;
+    int      mapping = kMapAuto;
+    int      closureOn = -1;   // -1: on for mixed, off for plain
+    double   uploadSeconds = 0.0, downloadSeconds = 0.0;
+    Timings      t;
+    ForceTimings ft;
+
+    bool readyForForce(int nn) const { return n == nn && token != 0; }
+};
+
+State& state();
+
+} // namespace csph

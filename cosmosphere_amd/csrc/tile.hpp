@@ -1,0 +1,60 @@
+/*
+ * tile.hpp -- staging of neighbour-leaf particles into LDS tiles, shared by the density,
+ * gradient and force kernels, and the layout of the device work counters.
+ */
+#pragma once
+
+#include "common.hpp"
+
+namespace csph
+{
+
+constexpr int kCounterSlots  = 256;
+constexpr int kCounterStride = 16;   // u64 per slot: one 128-byte line each
+
+// Stage the j-particles of list entries [lo, hi) as 64-slot tiles.  For every tile, each
+// lane with a slot calls load(slot, j); then consume(nt) runs with the tile complete.
+template<class Load, class Consume>
+__device__ inline void forEachTile(int* __restrict__ segStart, int* __restrict__ segJ,
+                                   const uint32_t* __restrict__ layout, const int* __restrict__ jList, uint32_t lo,
+                                   uint32_t hi, Load load, Consume consume)
+{
+    const int lane = laneId();
+    for (uint32_t e0 = lo; e0 < hi; e0 += kWave)
+    {
+        const uint32_t e = e0 + lane;
+        int js = 0, jc = 0;
+        if (e < hi)
+        {
+            const int jl = jList[e];
+            js = (int)layout[jl];
+            jc = (int)layout[jl + 1] - js;
+        }
+        const int incl  = waveInclusiveScan(jc);
+        const int total = __shfl(incl, kWave - 1, kWave);
+        segStart[lane] = incl - jc;
+        segJ[lane]     = js;
+        const int nseg = (int)min((uint32_t)kWave, hi - e0);
+        waveLdsSync();
+        for (int s0 = 0; s0 < total; s0 += kWave)
+        {
+            const int slot = s0 + lane;
+            if (slot < total)
+            {
+                int a = 0, b = nseg;   // last t with segStart[t] <= slot
+                while (b - a > 1)
+                {
+                    const int m = (a + b) >> 1;
+                    if (segStart[m] <= slot) a = m;
+                    else                     b = m;
+                }
+                load(lane, segJ[a] + slot - segStart[a]);
+            }
+            waveLdsSync();
+            consume(min(kWave, total - s0));
+            waveLdsSync();
+        }
+    }
+}
+
+} // namespace csph

@@ -1,0 +1,116 @@
+/*
+ * cosmosphere.h -- C ABI of the MI355X (gfx950) SPH density / smoothing-length and force
+ * solver.  Drop-in for the reference's libcosmoSPHere.a: the ten entry points below keep the
+ * reference's names, signatures, argument meaning, slot numbering and abort-on-misuse
+ * behaviour, so a Fortran `bind(C)` interface written for the reference links unchanged.
+ *
+ * Build products (cosmosphere_amd/lib/):
+ *   libcosmoSPHere.a / .so           M4 cubic kernel   (reference KERNEL=cubic)
+ *   libcosmoSPHere_quintic.a / .so   M6 quintic kernel (reference KERNEL=quintic)
+ *
+ * Conventions (as the reference):
+ *   - single device, single host thread, not re-entrant; every call returns when its work
+ *     is complete;
+ *   - misuse (wrong particle count, unknown slot, no ordering yet, force before density)
+ *     and unconverged solves print a "FATAL: ..." line on stderr and abort();
+ *   - particle data stay on the device between calls, in Hilbert order; transfers gather
+ *     on the way in and scatter on the way out, so the caller only ever sees its own order;
+ *   - a slot's host block is ncomp contiguous runs of n doubles.
+ */
+#ifndef COSMOSPHERE_H
+#define COSMOSPHERE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Slot ids, shared with Phantom's ibun_* (reference include/arrays.hpp:15-25). */
+enum CosmoSlot
+{
+    COSMO_POS       = 1, /* x, y, z                               */
+    COSMO_HSML      = 2, /* h                                     */
+    COSMO_VEL       = 3, /* vx, vy, vz                            */
+    COSMO_ACCEL     = 4, /* ax, ay, az                            */
+    COSMO_DENS_OUT  = 5, /* rho, d(rho)/d(h)                      */
+    COSMO_GRAD_OUT  = 6, /* div v, xi, d(div v)/dt                */
+    COSMO_THERMO    = 7, /* p/rho^2, c_s, alpha_AV, u             */
+    COSMO_FORCE_OUT = 8  /* fx, fy, fz, du/dt, vsigmax, div v     */
+};
+
+/* ---- the ten reference entry points ---------------------------------------------------- */
+
+/* Density / smoothing-length Newton-Raphson solve plus the gradient sweep, on the POS, HSML,
+ * VEL and ACCEL slots previously uploaded in the caller's order.  box = {xmin, xmax, ymin,
+ * ymax, zmin, zmax}, read only if periodic != 0.  hfact <= 0 selects the kernel default.
+ * Replaces reference src/dens_c_api.cu:47-124 (densityiterate_gpu_c). */
+void densityiterate_gpu_c(int n, double pmass, int periodic, const double* box, double tolh, double hfact);
+
+/* Support radius of the compiled kernel in units of h: 2 (cubic) or 3 (quintic).
+ * Replaces reference src/dens_c_api.cu:128-131. */
+double cosmo_kernel_radius(void);
+
+/* Force pass on the tree and h of the last density solve for this n; THERMO (and, for the
+ * corrector, VEL) uploaded with cosmo_upload_sorted.  Leaves FORCE_OUT on the device.
+ * Replaces reference src/force_c_api.cu:26-82 (force_gpu_c). */
+void force_gpu_c(int n, double pmass, double beta, double alphau, int disc_viscosity, int pdv_heating,
+                 int shock_heating);
+
+/* Size the particle-length device arrays for n particles (n <= 0: no-op; idempotent).
+ * Replaces reference src/arrays_c_api.cu:30-35. */
+void cosmo_arrays_init(int n);
+
+/* Components of a slot: 1->3, 2->1, 3->3, 4->3, 5->2, 6->3, 7->4, 8->6, other->0.
+ * Replaces reference src/arrays_c_api.cu:145-155. */
+int cosmo_slot_ncomp(int slot);
+
+/* Send a slot in the caller's order (before the solve that sorts it).
+ * Replaces reference src/arrays_c_api.cu:157-161. */
+void cosmo_upload(int slot, const double* host, int n);
+
+/* Send a slot and gather it into the device's order; aborts if no solve has built one.
+ * Replaces reference src/arrays_c_api.cu:163-167. */
+void cosmo_upload_sorted(int slot, const double* host, int n);
+
+/* Fetch a slot back in the caller's order; needs an ordering.
+ * Replaces reference src/arrays_c_api.cu:169-193. */
+void cosmo_download(int slot, double* host, int n);
+
+/* Best-effort page-locking hints; never fail.  Replaces reference src/pin_c_api.cu:22-34
+ * (the registration itself is not performed on this platform). */
+void cosmo_pin_host(void* ptr, size_t nbytes);
+void cosmo_unpin_host(void* ptr);
+
+/* ---- extensions (no counterpart in the reference's C ABI) ------------------------------- */
+
+/* The reference's C++ solveDensH(n, pmass, mode, withGradients, periodicBox, tolh, hfact)
+ * (include/density.hpp:85-91) with C linkage: mode 0 = FLAT_PARTICLE, 1 = WARP_PER_LEAF.
+ * Returns the Newton iterations run. */
+int cosmo_ext_solve_dens_h(int n, double pmass, int mode, int withGradients, int periodic, const double* box,
+                           double tolh, double hfact);
+
+/* Options: "precision" (0 = plain FP64, 1 = mixed FP32-offset form), "max_iter",
+ * "mapping" (0 auto, 1 wave per leaf, 2 flat), "closure" (-1 auto, 0 off, 1 on).
+ * The environment variable COSMO_PRECISION=mixed|fp64 sets the precision for
+ * densityiterate_gpu_c. */
+void cosmo_ext_set_option(const char* name, double value);
+
+/* Per-phase device seconds and work counters of the last solve / force pass. */
+double cosmo_ext_get_stat(const char* name);
+
+/* Copy a named device array to host memory (at most maxBytes); returns its element count,
+ * -1 for an unknown name.  For verification. */
+long long cosmo_ext_get_array(const char* name, void* dst, long long maxBytes);
+
+/* Keep / restore a device copy of the uploaded POS, HSML, VEL and ACCEL (repeated solves
+ * from identical inputs). */
+void cosmo_ext_stash_inputs(void);
+void cosmo_ext_restore_inputs(void);
+void cosmo_ext_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COSMOSPHERE_H */

@@ -1,0 +1,1218 @@
+/*
+ * sph_oracle.cpp -- FP64 CPU restatement of the cosmoSPHere density / force hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is linked by the tests, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the product
+ * library (cosmosphere_amd/csrc).  The product path must not call it.
+ *
+ * Every function restates the reference algorithm it names (file:line in the reference
+ * tree).  Cornerstone (exafmm/cornerstone-octree, unpinned version, not vendored in the
+ * reference) is restated from its published algorithm; where a piece of it cannot be
+ * pinned by a property test the header of that function says "parity unpinned".
+ *
+ * Pinning (tests/test_oracle_*.py):
+ *   kernel W, dW          closed forms, normalisation quadrature, continuity at q=1,2(,3)
+ *   Hilbert encode        bijection with the independent decoder, face adjacency of
+ *                         consecutive keys, prefix nesting  (Cornerstone bit pattern: unpinned)
+ *   sort / nAlive         numpy stable argsort
+ *   leaf tree             brute-force top-down canonical tree (split iff count > bucket)
+ *   j-lists               superset of brute-force neighbour leaves; DFS order vs a
+ *                         recursive restatement written in Python
+ *   NR density            brute-force O(N^2) sums; fixed point rho_SPH(h) = m (hfact/h)^3
+ *   gradients             exact-linear velocity field returns its matrix; xi(0) = 1
+ *   force                 momentum conservation, energy conservation, zero conductivity
+ *                         for uniform u
+ *
+ * Built with -ffp-contract=off so that every expression is evaluated as written; the
+ * HIP kernels evaluate the tree-geometry tests with contraction off as well, which is
+ * what makes keys, trees and j-lists comparable bit for bit.
+ */
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc
+{
+
+constexpr double kPi = 3.141592653589793;
+constexpr int kMaxLevel = 21;                       // 63-bit keys, 21 octal digits
+constexpr uint64_t kRootRange = uint64_t(1) << 63;  // nodeRange(0) = 8^21
+constexpr uint64_t kDeadKey = ~uint64_t(0);
+
+// ---------------------------------------------------------------------------------------
+// Smoothing kernels: include/kernel.hpp:35-47 (M4 cubic), :53-78 (M6 quintic),
+// constants :80-99.
+// ---------------------------------------------------------------------------------------
+struct KernelSpec
+{
+    int    quintic;
+    double rad, rad2, cnormk, hfactDefault;
+};
+
+inline KernelSpec kernelSpec(int quintic)
+{
+    if (quintic) return {1, 3.0, 9.0, 1.0 / (120.0 * kPi), 1.0};
+    return {0, 2.0, 4.0, 1.0 / kPi, 1.2};
+}
+
+inline void m4Kern(double q, double& w, double& dw)
+{
+    const double a = (2.0 - q > 0.0) ? (2.0 - q) : 0.0;
+    const double b = (1.0 - q > 0.0) ? (1.0 - q) : 0.0;
+    const double a2 = a * a, a3 = a2 * a, b2 = b * b, b3 = b2 * b;
+    w  = 0.25 * (a3 - 4.0 * b3);
+    dw = 0.25 * (-3.0 * a2 + 12.0 * b2);
+}
+
+inline void m6Kern(double q2, double q, double& w, double& dw)
+{
+    if (q < 1.0)
+    {
+        const double q4 = q2 * q2;
+        w  = -10.0 * q4 * q + 30.0 * q4 - 60.0 * q2 + 66.0;
+        dw = q * (-50.0 * q2 * q + 120.0 * q2 - 120.0);
+    }
+    else if (q < 2.0)
+    {
+        const double a = q - 3.0, b = q - 2.0, a2 = a * a, b2 = b * b;
+        w  = -(a2 * a2 * a) + 6.0 * (b2 * b2 * b);
+        dw = -5.0 * (a2 * a2) + 30.0 * (b2 * b2);
+    }
+    else if (q < 3.0)
+    {
+        const double a = q - 3.0, a2 = a * a;
+        w  = -(a2 * a2 * a);
+        dw = -5.0 * (a2 * a2);
+    }
+    else { w = 0.0; dw = 0.0; }
+}
+
+inline void kern(const KernelSpec& k, double q2, double q, double& w, double& dw)
+{
+    if (k.quintic) m6Kern(q2, q, w, dw);
+    else           m4Kern(q, w, dw);
+}
+
+// ---------------------------------------------------------------------------------------
+// Root box: src/tree.cu:46-73.  Cornerstone Box keeps lx = xmax - xmin and ilx = 1/lx.
+// ---------------------------------------------------------------------------------------
+struct Box
+{
+    double lo[3], hi[3], l[3], il[3];
+    int    periodic;
+};
+
+inline Box makeBox(const double b[6], int periodic)
+{
+    Box box;
+    for (int a = 0; a < 3; ++a)
+    {
+        box.lo[a] = b[2 * a];
+        box.hi[a] = b[2 * a + 1];
+        box.l[a]  = box.hi[a] - box.lo[a];
+        box.il[a] = 1.0 / box.l[a];
+    }
+    box.periodic = periodic;
+    return box;
+}
+
+// Open box over ALL n particles (dead included), padded x1.00001 on the largest span
+// and centred: src/tree.cu:56-71.
+inline Box openBox(int n, const double* x, const double* y, const double* z)
+{
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const double* p[3] = {x, y, z};
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < n; ++i)
+        {
+            mn[a] = std::min(mn[a], p[a][i]);
+            mx[a] = std::max(mx[a], p[a][i]);
+        }
+    const double span = std::max({mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]}) * 1.00001;
+    const double half = 0.5 * span;
+    double b[6];
+    for (int a = 0; a < 3; ++a)
+    {
+        const double c = 0.5 * (mn[a] + mx[a]);
+        b[2 * a]     = c - half;
+        b[2 * a + 1] = c + half;
+    }
+    return makeBox(b, 0);
+}
+
+// ---------------------------------------------------------------------------------------
+// Hilbert keys.  Cornerstone iHilbert / hilbert3D restated from their published algorithm
+// (SURVEY appendix A.2): walk levels 20..0, append mortonToHilbert[octant], complement the
+// remaining coordinate bits, then rotate (zi) or swap x/z (!yi).  Validity of the curve
+// (bijection, face adjacency, prefix nesting) is pinned by tests; the exact Cornerstone bit
+// pattern is parity unpinned (Cornerstone source absent).
+// ---------------------------------------------------------------------------------------
+inline uint64_t iHilbert(unsigned px, unsigned py, unsigned pz)
+{
+    static const unsigned m2h[8] = {0, 1, 3, 2, 7, 6, 4, 5};
+    uint64_t key = 0;
+    for (int level = kMaxLevel - 1; level >= 0; --level)
+    {
+        const unsigned xi = (px >> level) & 1u, yi = (py >> level) & 1u, zi = (pz >> level) & 1u;
+        key = (key << 3) + m2h[(xi << 2) | (yi << 1) | zi];
+        px ^= -(xi & ((!yi) | zi));
+        py ^= -((xi & (yi | zi)) | (yi & (!zi)));
+        pz ^= -((xi & (!yi) & (!zi)) | (yi & (!zi)));
+        if (zi)       { const unsigned t = px; px = py; py = pz; pz = t; }
+        else if (!yi) { const unsigned t = px; px = pz; pz = t; }
+    }
+    return key;
+}
+
+// Quantisation of hilbert3D (A.2): ix = int(floor(x*mx) - xmin*mx), mx = 2^21/lx,
+// clamped to 2^21-1.
+inline unsigned quantise(double x, double lo, double il)
+{
+    const double m  = 2097152.0 * il;
+    int          ix = (int)(std::floor(x * m) - lo * m);
+    if (ix > (1 << kMaxLevel) - 1) ix = (1 << kMaxLevel) - 1;
+    if (ix < 0) ix = 0;
+    return (unsigned)ix;
+}
+
+inline uint64_t hilbert3D(double x, double y, double z, const Box& b)
+{
+    return iHilbert(quantise(x, b.lo[0], b.il[0]), quantise(y, b.lo[1], b.il[1]),
+                    quantise(z, b.lo[2], b.il[2]));
+}
+
+// Decoder, written independently of the encoder: the encoder's running transform is held
+// as an axis permutation plus a complement mask and inverted level by level.
+inline void decodeHilbert(uint64_t key, unsigned out[3])
+{
+    static const unsigned h2m[8] = {0, 1, 3, 2, 6, 7, 5, 4};   // inverse of {0,1,3,2,7,6,4,5}
+    int      perm[3] = {0, 1, 2};   // transformed axis a reads original axis perm[a]
+    unsigned flip[3] = {0, 0, 0};   // and is complemented when flip[a] = 1
+    out[0] = out[1] = out[2] = 0;
+    for (int level = kMaxLevel - 1; level >= 0; --level)
+    {
+        const unsigned o  = h2m[(key >> (3 * level)) & 7u];
+        const unsigned t[3] = {(o >> 2) & 1u, (o >> 1) & 1u, o & 1u};
+        for (int a = 0; a < 3; ++a) out[perm[a]] |= ((t[a] ^ flip[a]) & 1u) << level;
+        const unsigned xi = t[0], yi = t[1], zi = t[2];
+        flip[0] ^= (xi & ((!yi) | zi));
+        flip[1] ^= ((xi & (yi | zi)) | (yi & (!zi)));
+        flip[2] ^= ((xi & (!yi) & (!zi)) | (yi & (!zi)));
+        if (zi)
+        {
+            const int p0 = perm[0]; const unsigned f0 = flip[0];
+            perm[0] = perm[1]; flip[0] = flip[1];
+            perm[1] = perm[2]; flip[1] = flip[2];
+            perm[2] = p0;      flip[2] = f0;
+        }
+        else if (!yi)
+        {
+            std::swap(perm[0], perm[2]);
+            std::swap(flip[0], flip[2]);
+        }
+    }
+}
+
+// Node geometry: prefix (start key, level) -> integer box (hilbertIBox) -> FP64 centre and
+// half-size (centerAndSize): centre = lo + (imax + imin) * (0.5 * u * l),
+// size = (imax - imin) * (0.5 * u * l), u = 2^-21.  include/tree.cuh:110-125.
+inline void nodeGeometry(uint64_t start, int level, const Box& b, double c[3], double s[3])
+{
+    unsigned ic[3];
+    decodeHilbert(start, ic);
+    const unsigned cube = (1u << kMaxLevel) >> level;
+    const unsigned mask = ~(cube - 1u);
+    for (int a = 0; a < 3; ++a)
+    {
+        const unsigned imin = ic[a] & mask, imax = imin + cube;
+        const double   half = 0.5 * (1.0 / 2097152.0) * b.l[a];
+        c[a] = b.lo[a] + double(int(imax + imin)) * half;
+        s[a] = double(int(imax - imin)) * half;
+    }
+}
+
+// Cornerstone minDistance (box-box), SURVEY A.7: d = |cb - ca| - sa - sb; d = (d + |d|)/2.
+// Periodic: nearest image of cb - ca first (applyPbc, rint); parity unpinned.
+inline double minDist2(const double ac[3], const double as[3], const double bc[3],
+                       const double bs[3], const Box& b)
+{
+    double d[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        double dx = bc[a] - ac[a];
+        if (b.periodic) dx -= b.l[a] * std::rint(dx * b.il[a]);
+        double v = std::fabs(dx) - as[a] - bs[a];
+        v += std::fabs(v);
+        v *= 0.5;
+        d[a] = v;
+    }
+    return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+// Pair separation at its nearest image: include/tree.cuh:55-84.
+inline void nearestImage(double& dx, double& dy, double& dz, const Box& b)
+{
+    if (!b.periodic) return;
+    if (std::fabs(dx) > 0.5 * b.l[0]) dx = dx - b.l[0] * std::copysign(1.0, dx);
+    if (std::fabs(dy) > 0.5 * b.l[1]) dy = dy - b.l[1] * std::copysign(1.0, dy);
+    if (std::fabs(dz) > 0.5 * b.l[2]) dz = dz - b.l[2] * std::copysign(1.0, dz);
+}
+
+// ---------------------------------------------------------------------------------------
+// The octree as an explicit pointer tree (deliberately a different data structure from the
+// GPU's flat children-block layout).
+// ---------------------------------------------------------------------------------------
+struct Node
+{
+    uint64_t start;
+    int      level;
+    int      child[8];   // node ids, -1 for a leaf
+    int      leaf;       // leaf index, -1 for an internal node
+    double   c[3], s[3];
+    unsigned first, count;
+};
+
+struct State
+{
+    KernelSpec k;
+    int        n = 0, nAlive = 0, bucket = 64;
+    Box        box{};
+    int        haveOrder = 0;
+
+    // particle data, Hilbert order after build()
+    std::vector<double> x, y, z, h, vx, vy, vz, ax, ay, az;
+    std::vector<double> rho, gradh, divv, xi, ddivvdt;
+    std::vector<double> pro2, cs, alpha, u;
+    std::vector<double> fx, fy, fz, f4, vsig, divvF;
+    std::vector<uint64_t> keys;
+    std::vector<int>      order;
+
+    // tree
+    std::vector<Node>     nodes;
+    std::vector<uint64_t> leafKey;     // csTree, L+1
+    std::vector<int>      leafLevel;   // L
+    std::vector<unsigned> layout;      // L+1
+    std::vector<int>      leafNode;    // leaf -> node id
+    std::vector<int>      particleLeaf;
+    std::vector<uint64_t> internalStart; // internal nodes sorted by (level, key)
+    std::vector<int>      internalLevel;
+
+    std::vector<double> hmaxLeaf, hmaxNode;
+    std::vector<int>    jOffset, jCount, jList;
+
+    // NR statistics
+    int    itersRun = 0, unconverged = 0;
+    std::vector<int> activePerIter;
+    std::vector<double> pairsPerIter, candPerIter;
+};
+
+int nLeaves(const State& s) { return (int)s.leafLevel.size(); }
+
+// Canonical leaf tree: Cornerstone updateOctreeGpu iterated to convergence
+// (src/tree.cu:121-136) has the unique fixed point "split iff count > bucket and level < 21"
+// over the live keys.  Built here top-down by recursion.
+static int buildNode(State& s, uint64_t start, int level, unsigned a, unsigned b)
+{
+    const int id = (int)s.nodes.size();
+    s.nodes.push_back(Node{});
+    Node nd{};
+    nd.start = start;
+    nd.level = level;
+    nd.first = a;
+    nd.count = b - a;
+    for (int o = 0; o < 8; ++o) nd.child[o] = -1;
+    nd.leaf = -1;
+    nodeGeometry(start, level, s.box, nd.c, nd.s);
+    if ((b - a) > (unsigned)s.bucket && level < kMaxLevel)
+    {
+        const uint64_t range = uint64_t(1) << (3 * (kMaxLevel - 1 - level));
+        unsigned lo = a;
+        for (int o = 0; o < 8; ++o)
+        {
+            const uint64_t cs = start + uint64_t(o) * range, ce = cs + range;
+            const unsigned hi =
+                (unsigned)(std::lower_bound(s.keys.begin() + lo, s.keys.begin() + b, ce) - s.keys.begin());
+            nd.child[o] = buildNode(s, cs, level + 1, lo, hi);
+            lo = hi;
+        }
+    }
+    else
+    {
+        nd.leaf = (int)s.leafLevel.size();
+        s.leafKey.push_back(start);
+        s.leafLevel.push_back(level);
+        s.layout.push_back(a);
+        s.leafNode.push_back(id);
+    }
+    s.nodes[id] = nd;
+    return id;
+}
+
+// bbox, keys, stable sort, nAlive, permute, tree, geometry, maps: src/tree.cu:28-200.
+void build(State& s, int periodic, const double* pbox)
+{
+    const int n = s.n;
+    s.box = periodic ? makeBox(pbox, 1) : openBox(n, s.x.data(), s.y.data(), s.z.data());
+
+    // keys: include/tree.cuh:89-105 (dead particles, h <= 0, get ~0)
+    std::vector<uint64_t> k(n);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i)
+        k[i] = (s.h[i] > 0.0) ? hilbert3D(s.x[i], s.y[i], s.z[i], s.box) : kDeadKey;
+
+    // stable sort of (key, iota): src/tree.cu:90-91
+    s.order.resize(n);
+    std::iota(s.order.begin(), s.order.end(), 0);
+    std::stable_sort(s.order.begin(), s.order.end(), [&](int a, int b) { return k[a] < k[b]; });
+    s.keys.resize(n);
+    for (int i = 0; i < n; ++i) s.keys[i] = k[s.order[i]];
+    s.nAlive = (int)(std::lower_bound(s.keys.begin(), s.keys.end(), kDeadKey) - s.keys.begin());
+
+    auto permute = [&](std::vector<double>& v) {
+        if ((int)v.size() != n) return;
+        std::vector<double> t(n);
+        for (int i = 0; i < n; ++i) t[i] = v[s.order[i]];
+        v.swap(t);
+    };
+    for (auto* v : {&s.x, &s.y, &s.z, &s.h, &s.vx, &s.vy, &s.vz, &s.ax, &s.ay, &s.az}) permute(*v);
+    s.haveOrder = 1;
+
+    // tree
+    s.nodes.clear();
+    s.leafKey.clear();
+    s.leafLevel.clear();
+    s.layout.clear();
+    s.leafNode.clear();
+    buildNode(s, 0, 0, 0, (unsigned)s.nAlive);
+    s.leafKey.push_back(kRootRange);
+    s.layout.push_back((unsigned)s.nAlive);
+
+    // internal nodes in (level, key) order
+    std::vector<std::pair<int, uint64_t>> in;
+    for (const Node& nd : s.nodes)
+        if (nd.leaf < 0) in.push_back({nd.level, nd.start});
+    std::sort(in.begin(), in.end());
+    s.internalStart.resize(in.size());
+    s.internalLevel.resize(in.size());
+    for (size_t i = 0; i < in.size(); ++i)
+    {
+        s.internalLevel[i] = in[i].first;
+        s.internalStart[i] = in[i].second;
+    }
+
+    // particle -> leaf: include/tree.cuh:165-174
+    s.particleLeaf.assign(n, -1);
+    const int L = nLeaves(s);
+    for (int l = 0; l < L; ++l)
+        for (unsigned j = s.layout[l]; j < s.layout[l + 1]; ++j) s.particleLeaf[j] = l;
+    s.hmaxLeaf.assign(L, 0.0);
+    s.hmaxNode.assign(s.nodes.size(), 0.0);
+}
+
+// hmax_leaf = max finite h over the leaf: include/tree.cuh:177-191
+void hmaxLeaves(State& s, const std::vector<int>& leaves)
+{
+#pragma omp parallel for schedule(static)
+    for (size_t a = 0; a < leaves.size(); ++a)
+    {
+        const int l = leaves[a];
+        double    m = 0.0;
+        for (unsigned j = s.layout[l]; j < s.layout[l + 1]; ++j)
+            if (std::isfinite(s.h[j])) m = std::fmax(m, s.h[j]);
+        s.hmaxLeaf[l] = m;
+    }
+}
+
+// hmax per node, children first: include/tree.cuh:196-217, src/force.cu:43-60
+static double upsweep(State& s, int id)
+{
+    Node& nd = s.nodes[id];
+    double m;
+    if (nd.leaf >= 0)
+    {
+        const double hm = s.hmaxLeaf[nd.leaf];
+        m = (std::isfinite(hm) && hm > 0.0) ? hm : 0.0;
+    }
+    else
+    {
+        m = 0.0;
+        for (int o = 0; o < 8; ++o) m = std::fmax(m, upsweep(s, nd.child[o]));
+    }
+    s.hmaxNode[id] = m;
+    return m;
+}
+
+// nodeInRange<Symmetric, Periodic>: include/tree.cuh:228-252
+static inline bool inRange(const State& s, bool sym, const double ic[3], const double isz[3],
+                           const double ihalf[3], double radHi, int node)
+{
+    const Node& nd = s.nodes[node];
+    const double d2 = minDist2(ic, sym ? isz : ihalf, nd.c, nd.s, s.box);
+    if (sym)
+    {
+        const double rs = std::fmax(radHi, s.k.rad * s.hmaxNode[node]);
+        return d2 <= rs * rs;
+    }
+    return d2 <= 0.0;
+}
+
+// DFS j-leaf list of one i-leaf, in the reference's order: root test, then for each popped
+// node the children in key (octant) order, leaves appended, internal nodes pushed, LIFO.
+// include/tree.cuh:258-353
+static void walk(const State& s, bool sym, int iLeaf, std::vector<int>& out)
+{
+    out.clear();
+    const Node& in = s.nodes[s.leafNode[iLeaf]];
+    double hmax = s.hmaxLeaf[iLeaf];
+    if (!std::isfinite(hmax) || hmax <= 0.0) hmax = 0.0;
+    const double radHi = s.k.rad * hmax;
+    const double ihalf[3] = {in.s[0] + radHi, in.s[1] + radHi, in.s[2] + radHi};
+    if (!inRange(s, sym, in.c, in.s, ihalf, radHi, 0)) return;
+    if (s.nodes[0].leaf >= 0) { out.push_back(s.nodes[0].leaf); return; }
+    std::vector<int> stack;
+    stack.push_back(-1);
+    int node = 0;
+    do
+    {
+        const Node& nd = s.nodes[node];
+        for (int o = 0; o < 8; ++o)
+        {
+            const int c = nd.child[o];
+            if (!inRange(s, sym, in.c, in.s, ihalf, radHi, c)) continue;
+            if (s.nodes[c].leaf >= 0) out.push_back(s.nodes[c].leaf);
+            else                      stack.push_back(c);
+        }
+        node = stack.back();
+        stack.pop_back();
+    } while (node != -1);
+}
+
+// CSR j-lists over all leaves: include/tree.cuh:362-389 (symmetric: src/force.cu:19-88)
+void buildLists(State& s, bool sym)
+{
+    const int L = nLeaves(s);
+    if (sym) upsweep(s, 0);
+    std::vector<std::vector<int>> lists(L);
+#pragma omp parallel
+    {
+        std::vector<int> tmp;
+#pragma omp for schedule(dynamic, 16)
+        for (int l = 0; l < L; ++l)
+        {
+            walk(s, sym, l, tmp);
+            lists[l] = tmp;
+        }
+    }
+    s.jOffset.assign(L + 1, 0);
+    s.jCount.assign(L, 0);
+    for (int l = 0; l < L; ++l)
+    {
+        s.jCount[l]      = (int)lists[l].size();
+        s.jOffset[l + 1] = s.jOffset[l] + s.jCount[l];
+    }
+    s.jList.resize(s.jOffset[L]);
+    for (int l = 0; l < L; ++l) std::copy(lists[l].begin(), lists[l].end(), s.jList.begin() + s.jOffset[l]);
+}
+
+// Density sums of particle i over its leaf's list: src/density_base.cu:262-292
+struct DensSums
+{
+    double w = 0.0, g = 0.0;
+    long   cand = 0, acc = 0;
+};
+
+static DensSums densitySums(const State& s, int i, double hi, const int* list, int nj)
+{
+    DensSums r;
+    const double hi_sq_inv = 1.0 / (hi * hi);
+    for (int jl = 0; jl < nj; ++jl)
+    {
+        const int jLeaf = list[jl];
+        for (unsigned j = s.layout[jLeaf]; j < s.layout[jLeaf + 1]; ++j)
+        {
+            double dx = s.x[i] - s.x[j], dy = s.y[i] - s.y[j], dz = s.z[i] - s.z[j];
+            nearestImage(dx, dy, dz, s.box);
+            const double q2 = (dx * dx + dy * dy + dz * dz) * hi_sq_inv;
+            ++r.cand;
+            if (q2 < s.k.rad2)
+            {
+                const double q = std::sqrt(q2);
+                double w, dw;
+                kern(s.k, q2, q, w, dw);
+                r.w += w;
+                r.g += -q * dw - 3.0 * w;
+                ++r.acc;
+            }
+        }
+    }
+    return r;
+}
+
+// Newton-Raphson update: src/density_base.cu:294-333
+static void nrUpdate(const State& s, double pmass, double tolh, double hfact, double wsum,
+                     double gsum, double& h, double& rho, double& gradh, int& conv)
+{
+    const double hi = h, hi_old = h;
+    const double hi1 = 1.0 / hi, hi31 = hi1 * hi1 * hi1, hi41 = hi31 * hi1;
+    const double rho_i  = wsum * s.k.cnormk * pmass * hi31;
+    const double grad_i = gsum * s.k.cnormk * pmass * hi41;
+    if (!(rho_i > 0.0))
+    {
+        rho = 0.0; gradh = 0.0; conv = 0;
+        return;
+    }
+    const double rhoh_i  = pmass * (hfact * hi1) * (hfact * hi1) * (hfact * hi1);
+    const double funci   = rhoh_i - rho_i;
+    const double dhdrhoi = -hi / (3.0 * rhoh_i);
+    const double omegai  = 1.0 - dhdrhoi * grad_i;
+    const double safe    = (omegai > 0.0) ? omegai : std::fabs(omegai + 1e-300);
+    const double hraw    = hi - funci * dhdrhoi / safe;
+    double hnew;
+    if      (hraw > 1.2 * hi) hnew = 1.2 * hi;
+    else if (hraw < 0.8 * hi) hnew = 0.8 * hi;
+    else                      hnew = hraw;
+    rho   = rho_i;
+    gradh = grad_i;
+    h     = hnew;
+    conv  = (std::fabs((hnew - hi) / hi_old) < tolh) ? 1 : 0;
+}
+
+// rho and d(rho)/d(h) at the current h with no update (the closure of the gradient sweep,
+// src/density_base.cu:635-643), for every live particle.
+void closure(State& s, double pmass)
+{
+    const int L = nLeaves(s);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int l = 0; l < L; ++l)
+        for (unsigned i = s.layout[l]; i < s.layout[l + 1]; ++i)
+        {
+            const DensSums r = densitySums(s, i, s.h[i], s.jList.data() + s.jOffset[l], s.jCount[l]);
+            const double hi1 = 1.0 / s.h[i], hi31 = hi1 * hi1 * hi1, hi41 = hi31 * hi1;
+            s.rho[i]   = r.w * s.k.cnormk * pmass * hi31;
+            s.gradh[i] = r.g * s.k.cnormk * pmass * hi41;
+        }
+}
+
+// Cullen & Dehnen xi limiter: src/density_base.cu:511-527 (components rounded to float)
+static double xiLimiter(const double g[9])
+{
+    const double dvxdx = (float)g[0], dvxdy = (float)g[1], dvxdz = (float)g[2];
+    const double dvydx = (float)g[3], dvydy = (float)g[4], dvydz = (float)g[5];
+    const double dvzdx = (float)g[6], dvzdy = (float)g[7], dvzdz = (float)g[8];
+    const double divv = dvxdx + dvydy + dvzdz;
+    const double cx = dvzdy - dvydz, cy = dvxdz - dvzdx, cz = dvydx - dvxdy;
+    const double mdiv = std::fmax(-divv, 0.0);
+    const double fac = mdiv * mdiv;
+    const double tr = cx * cx + cy * cy + cz * cz;
+    return (fac + tr > 2.220446049250313e-16) ? fac / (fac + tr) : 1.0;
+}
+
+// Post-convergence gradient sweep: src/density_base.cu:529-714
+static void gradientParticle(State& s, int i, double pmass)
+{
+    if (!(s.h[i] > 0.0))
+    {
+        s.divv[i] = 0.0; s.ddivvdt[i] = 0.0; s.xi[i] = 1.0;
+        return;
+    }
+    const double xi = s.x[i], yi = s.y[i], zi = s.z[i];
+    const double vxi = s.vx[i], vyi = s.vy[i], vzi = s.vz[i];
+    const double axi = s.ax[i], ayi = s.ay[i], azi = s.az[i];
+    const double hi = s.h[i], hsi = 1.0 / (hi * hi);
+    double rhoi = 0, gradhi = 0, divs = 0, dv[9] = {0}, da[9] = {0};
+    double rxx = 0, rxy = 0, rxz = 0, ryy = 0, ryz = 0, rzz = 0;
+    const int l = s.particleLeaf[i];
+    for (int jl = 0; jl < s.jCount[l]; ++jl)
+    {
+        const int jLeaf = s.jList[s.jOffset[l] + jl];
+        for (unsigned j = s.layout[jLeaf]; j < s.layout[jLeaf + 1]; ++j)
+        {
+            double dx = xi - s.x[j], dy = yi - s.y[j], dz = zi - s.z[j];
+            nearestImage(dx, dy, dz, s.box);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double q2 = r2 * hsi;
+            if (q2 >= s.k.rad2) continue;
+            const double q = std::sqrt(q2);
+            double w, dw;
+            kern(s.k, q2, q, w, dw);
+            rhoi += w;
+            gradhi += -q * dw - 3.0 * w;
+            const double rij = std::sqrt(r2), rij1 = 1.0 / (rij + 2.220446049250313e-16);
+            const double rg = rij1 * dw;
+            const double rx = dx * rg * pmass, ry = dy * rg * pmass, rz = dz * rg * pmass;
+            const double dvx = vxi - s.vx[j], dvy = vyi - s.vy[j], dvz = vzi - s.vz[j];
+            divs += dvx * rx + dvy * ry + dvz * rz;
+            dv[0] += dvx * rx; dv[1] += dvx * ry; dv[2] += dvx * rz;
+            dv[3] += dvy * rx; dv[4] += dvy * ry; dv[5] += dvy * rz;
+            dv[6] += dvz * rx; dv[7] += dvz * ry; dv[8] += dvz * rz;
+            const double dax = axi - s.ax[j], day = ayi - s.ay[j], daz = azi - s.az[j];
+            da[0] += dax * rx; da[1] += dax * ry; da[2] += dax * rz;
+            da[3] += day * rx; da[4] += day * ry; da[5] += day * rz;
+            da[6] += daz * rx; da[7] += daz * ry; da[8] += daz * rz;
+            rxx -= dx * rx; rxy -= dx * ry; rxz -= dx * rz;
+            ryy -= dy * ry; ryz -= dy * rz; rzz -= dz * rz;
+        }
+    }
+    const double hi1 = 1.0 / hi, hi31 = hi1 * hi1 * hi1, hi41 = hi31 * hi1;
+    const double rho_i = rhoi * s.k.cnormk * pmass * hi31;
+    const double grad_i = gradhi * s.k.cnormk * pmass * hi41;
+    s.rho[i] = rho_i;
+    s.gradh[i] = grad_i;
+    if (!(rho_i > 0.0))
+    {
+        s.divv[i] = 0.0; s.ddivvdt[i] = 0.0; s.xi[i] = 1.0;
+        return;
+    }
+    const double omega = 1.0 + (hi / (3.0 * rho_i)) * grad_i;
+    const double oinv = (omega > 0.0) ? (1.0 / omega) : 1.0;
+    const double term = s.k.cnormk * oinv * hi41 / rho_i;
+    const double denom = rxx * ryy * rzz + 2.0 * rxy * rxz * ryz - rxx * ryz * ryz - ryy * rxz * rxz - rzz * rxy * rxy;
+    const double rm0 = ryy * rzz - ryz * ryz, rm1 = rxz * ryz - rzz * rxy, rm2 = rxy * ryz - rxz * ryy;
+    const double rm3 = rzz * rxx - rxz * rxz, rm4 = rxy * rxz - rxx * ryz, rm5 = rxx * ryy - rxy * rxy;
+    s.divv[i] = -divs * term;
+    double g[9], div_a;
+    if (std::fabs(denom) > 2.2250738585072014e-308)
+    {
+        const double dd = 1.0 / denom;
+        for (int row = 0; row < 3; ++row)
+        {
+            const double sx = dv[3 * row], sy = dv[3 * row + 1], sz = dv[3 * row + 2];
+            g[3 * row]     = -(sx * rm0 + sy * rm1 + sz * rm2) * dd;
+            g[3 * row + 1] = -(sx * rm1 + sy * rm3 + sz * rm4) * dd;
+            g[3 * row + 2] = -(sx * rm2 + sy * rm4 + sz * rm5) * dd;
+        }
+        double gd = 0.0;
+        for (int row = 0; row < 3; ++row)
+        {
+            const double sx = da[3 * row], sy = da[3 * row + 1], sz = da[3 * row + 2];
+            const double gax = (sx * rm0 + sy * rm1 + sz * rm2) * dd;
+            const double gay = (sx * rm1 + sy * rm3 + sz * rm4) * dd;
+            const double gaz = (sx * rm2 + sy * rm4 + sz * rm5) * dd;
+            gd += (row == 0) ? gax : ((row == 1) ? gay : gaz);
+        }
+        div_a = -gd;
+    }
+    else
+    {
+        for (int c = 0; c < 9; ++c) g[c] = -dv[c] * term;
+        div_a = -term * (da[0] + da[4] + da[8]);
+    }
+    s.xi[i] = xiLimiter(g);
+    s.ddivvdt[i] = div_a - (g[0] * g[0] + g[4] * g[4] + g[8] * g[8] + 2.0 * (g[1] * g[3] + g[2] * g[6] + g[5] * g[7]));
+}
+
+// The solve: src/density_base.cu:720-1015.  mode 0 = FLAT_PARTICLE (only unconverged
+// particles are updated), mode 1 = WARP_PER_LEAF (every particle of an active leaf is
+// recomputed, src/density_base.cu:361-364).  Returns the number of unconverged particles.
+int densitySolve(State& s, double pmass, double tolh, double hfact, int maxIter, int mode,
+                 int withGradients)
+{
+    if (!(hfact > 0.0)) hfact = s.k.hfactDefault;
+    const int n = s.n, L = nLeaves(s);
+    s.rho.assign(n, 0.0);
+    s.gradh.assign(n, 0.0);
+    std::vector<int> conv(n, 0);
+    std::vector<int> active(s.nAlive), leaves(L);
+    std::iota(active.begin(), active.end(), 0);
+    std::iota(leaves.begin(), leaves.end(), 0);
+    s.itersRun = 0;
+    s.activePerIter.clear();
+    s.pairsPerIter.clear();
+    s.candPerIter.clear();
+
+    std::vector<char> isActive(n, 0);
+    for (int iter = 0; iter < maxIter && !active.empty(); ++iter)
+    {
+        hmaxLeaves(s, leaves);
+        // lists of the active leaves (the reference rebuilds all leaves; the lists of
+        // converged leaves are built but never read)
+        std::vector<std::vector<int>> lists(leaves.size());
+#pragma omp parallel
+        {
+            std::vector<int> tmp;
+#pragma omp for schedule(dynamic, 16)
+            for (size_t a = 0; a < leaves.size(); ++a)
+            {
+                walk(s, false, leaves[a], tmp);
+                lists[a] = tmp;
+            }
+        }
+        std::vector<int> slot(L, -1);
+        for (size_t a = 0; a < leaves.size(); ++a) slot[leaves[a]] = (int)a;
+
+        std::fill(isActive.begin(), isActive.end(), 0);
+        for (int i : active) isActive[i] = 1;
+
+        double pairs = 0, cand = 0;
+        // work items: FLAT = the active particles; WARP_PER_LEAF = every particle of an
+        // active leaf
+        std::vector<int> work;
+        if (mode == 0) work = active;
+        else
+            for (int l : leaves)
+                for (unsigned i = s.layout[l]; i < s.layout[l + 1]; ++i) work.push_back((int)i);
+        // sums first, then updates: the sums read positions only, never h of a neighbour
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : pairs, cand)
+        for (size_t w = 0; w < work.size(); ++w)
+        {
+            const int i = work[w];
+            const auto& li = lists[slot[s.particleLeaf[i]]];
+            const DensSums r = densitySums(s, i, s.h[i], li.data(), (int)li.size());
+            pairs += (double)r.acc;
+            cand += (double)r.cand;
+            nrUpdate(s, pmass, tolh, hfact, r.w, r.g, s.h[i], s.rho[i], s.gradh[i], conv[i]);
+        }
+        s.activePerIter.push_back((int)active.size());
+        s.pairsPerIter.push_back(pairs);
+        s.candPerIter.push_back(cand);
+        std::vector<int> next;
+        for (int i : active)
+            if (!conv[i]) next.push_back(i);
+        active.swap(next);
+        s.itersRun = iter + 1;
+        if (active.empty()) break;
+        leaves.clear();
+        for (int i : active) leaves.push_back(s.particleLeaf[i]);
+        leaves.erase(std::unique(leaves.begin(), leaves.end()), leaves.end());
+    }
+    s.unconverged = (int)active.size();
+
+    if (withGradients)
+    {
+        std::vector<int> all(L);
+        std::iota(all.begin(), all.end(), 0);
+        hmaxLeaves(s, all);
+        buildLists(s, false);
+        s.divv.assign(n, 0.0);
+        s.xi.assign(n, 1.0);
+        s.ddivvdt.assign(n, 0.0);
+        if ((int)s.vx.size() != n) { s.vx.assign(n, 0.0); s.vy.assign(n, 0.0); s.vz.assign(n, 0.0); }
+        if ((int)s.ax.size() != n) { s.ax.assign(n, 0.0); s.ay.assign(n, 0.0); s.az.assign(n, 0.0); }
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < n; ++i) gradientParticle(s, i, pmass);
+    }
+    return s.unconverged;
+}
+
+// ---------------------------------------------------------------------------------------
+// Force pass: src/force.cu:92-137 (prep) and :145-383 (pair sums)
+// ---------------------------------------------------------------------------------------
+void force(State& s, double pmass, double hfact, double beta, double alphau, int disc, int pdv,
+           int shock)
+{
+    const int n = s.n;
+    if (!(hfact > 0.0)) hfact = s.k.hfactDefault;
+    std::vector<double> hsqinv(n), hinv(n), rhoh(n), rho1(n), grkfac(n), pres(n), auterm(n), divfac(n);
+    for (int i = 0; i < n; ++i)
+    {
+        const double hi = s.h[i];
+        if (!(hi > 0.0))
+        {
+            hsqinv[i] = hinv[i] = rhoh[i] = rho1[i] = grkfac[i] = pres[i] = auterm[i] = divfac[i] = 0.0;
+            continue;
+        }
+        const double hs = 1.0 / (hi * hi), h4 = hs * hs, hfoh = hfact / hi;
+        const double rho = pmass * hfoh * hfoh * hfoh;
+        const double dhdrho = -hi / (3.0 * rho);
+        const double omega = 1.0 - dhdrho * s.gradh[i];
+        const double oinv = (omega > 0.0) ? (1 / omega) : 1.0;
+        hsqinv[i] = hs;
+        hinv[i] = 1.0 / hi;
+        rhoh[i] = rho;
+        rho1[i] = 1.0 / rho;
+        grkfac[i] = h4 * s.k.cnormk * oinv;
+        pres[i] = s.pro2[i] * rho * rho;
+        auterm[i] = 0.5 * pmass * (1.0 / rho) * alphau;
+        divfac[i] = s.k.cnormk * oinv * h4 / rho;
+    }
+    s.fx.assign(n, 0); s.fy.assign(n, 0); s.fz.assign(n, 0);
+    s.f4.assign(n, 0); s.vsig.assign(n, 0); s.divvF.assign(n, 0);
+    const bool full = pdv && shock;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i)
+    {
+        const double hi = s.h[i];
+        if (!(hi > 0.0)) continue;
+        const double xi = s.x[i], yi = s.y[i], zi = s.z[i];
+        const double hsi = hsqinv[i], rhoi = rhoh[i], gki = grkfac[i];
+        const double pro2i = s.pro2[i], vwi = s.cs[i], ali = s.alpha[i], eni = s.u[i];
+        const double rho1i = rho1[i], pri = pres[i], aui = auterm[i];
+        double ix = 0, iy = 0, iz = 0, jx = 0, jy = 0, jz = 0, f4s = 0, vmax = 0, divs = 0;
+        const int l = s.particleLeaf[i];
+        for (int jl = 0; jl < s.jCount[l]; ++jl)
+        {
+            const int jLeaf = s.jList[s.jOffset[l] + jl];
+            for (unsigned j = s.layout[jLeaf]; j < s.layout[jLeaf + 1]; ++j)
+            {
+                double dx = xi - s.x[j], dy = yi - s.y[j], dz = zi - s.z[j];
+                nearestImage(dx, dy, dz, s.box);
+                const double dr2 = dx * dx + dy * dy + dz * dz;
+                if (!(dr2 > 0.0)) continue;
+                const double hj = s.h[j];
+                {
+                    const double rmax2 = s.k.rad2 * std::fmax(hi * hi, hj * hj) * (1.0 + 1e-10);
+                    if (dr2 >= rmax2) continue;
+                }
+                const double qi2 = dr2 * hsi, qj2 = dr2 * hsqinv[j];
+                const double dr = std::sqrt(dr2), rij1 = 1.0 / dr;
+                const double rx = dx * rij1, ry = dy * rij1, rz = dz * rij1;
+                const double dvx = s.vx[i] - s.vx[j], dvy = s.vy[i] - s.vy[j], dvz = s.vz[i] - s.vz[j];
+                const double projv = dvx * rx + dvy * ry + dvz * rz;
+                const double rhoj = rhoh[j], rho1j = rho1[j], pro2j = s.pro2[j];
+                const double vwj = s.cs[j], alj = s.alpha[j], enj = s.u[j], prj = pres[j], auj = auterm[j];
+                const double vsigi = std::fmax(vwi - beta * projv, 0.0);
+                const double vsigavi = std::fmax(ali * vwi - beta * projv, 0.0);
+                const double vsigj = std::fmax(vwj - beta * projv, 0.0);
+                const double vsigavj = std::fmax(alj * vwj - beta * projv, 0.0);
+                const double pvmax = std::fmax(vsigi, vsigj);
+                double qr2i = 0.0, qr2j = 0.0;
+                const double denij = eni - enj;
+                const double vsigu = std::sqrt(std::fabs(pri - prj) * (2.0 / (rhoi + rhoj)));
+                if (disc)
+                {
+                    const double hori = hi * rij1, horj = hj * rij1;
+                    if (projv < 0.0)
+                    {
+                        qr2i = -0.5 * rho1i * (ali * vwi - beta * projv) * hori * projv;
+                        qr2j = -0.5 * rho1j * (alj * vwj - beta * projv) * horj * projv;
+                    }
+                    else
+                    {
+                        qr2i = -0.5 * rho1i * ali * vwi * hori * projv;
+                        qr2j = -0.5 * rho1j * alj * vwj * horj * projv;
+                    }
+                }
+                else if (projv < 0.0)
+                {
+                    qr2i = -0.5 * rho1i * vsigavi * projv;
+                    qr2j = -0.5 * rho1j * vsigavj * projv;
+                }
+                if (qi2 < s.k.rad2)
+                {
+                    vmax = std::fmax(vmax, pvmax);
+                    double w, dw;
+                    const double q = dr * hinv[i];
+                    kern(s.k, qi2, q, w, dw);
+                    divs += pmass * dw * projv;
+                    const double gk = dw * gki;
+                    const double gp = pmass * (pro2i + qr2i) * gk;
+                    ix += -gp * rx; iy += -gp * ry; iz += -gp * rz;
+                    const double pdvt = pmass * pro2i * projv * gk;
+                    const double diss = disc ? -0.5 * pmass * rho1i * ali * vwi * (hi * rij1) * projv * projv * gk
+                                             : pmass * qr2i * projv * gk;
+                    const double dend = vsigu * denij * aui * gk;
+                    if (full) { f4s += pdvt; f4s += diss; }
+                    else
+                    {
+                        if (pdv) f4s += pdvt;
+                        if (shock) f4s += diss;
+                    }
+                    f4s += dend;
+                }
+                if (qj2 < s.k.rad2)
+                {
+                    vmax = std::fmax(vmax, pvmax);
+                    double w, dw;
+                    const double q = dr * hinv[j];
+                    kern(s.k, qj2, q, w, dw);
+                    const double gk = dw * grkfac[j];
+                    const double gp = pmass * (pro2j + qr2j) * gk;
+                    jx -= gp * rx; jy -= gp * ry; jz -= gp * rz;
+                    f4s += vsigu * denij * auj * gk;
+                }
+            }
+        }
+        s.fx[i] = ix + jx; s.fy[i] = iy + jy; s.fz[i] = iz + jz;
+        s.f4[i] = f4s; s.vsig[i] = vmax;
+        s.divvF[i] = -divs * divfac[i];
+    }
+}
+
+} // namespace orc
+
+// =========================================================================================
+// C interface for the tests (ctypes)
+// =========================================================================================
+using orc::State;
+
+extern "C" {
+
+void orc_kernel(int quintic, int n, const double* q, double* w, double* dw)
+{
+    const orc::KernelSpec k = orc::kernelSpec(quintic);
+    for (int i = 0; i < n; ++i) orc::kern(k, q[i] * q[i], q[i], w[i], dw[i]);
+}
+
+void orc_kernel_consts(int quintic, double* out4)
+{
+    const orc::KernelSpec k = orc::kernelSpec(quintic);
+    out4[0] = k.rad; out4[1] = k.rad2; out4[2] = k.cnormk; out4[3] = k.hfactDefault;
+}
+
+void orc_hilbert_encode(int n, const unsigned* ix, const unsigned* iy, const unsigned* iz, uint64_t* keys)
+{
+    for (int i = 0; i < n; ++i) keys[i] = orc::iHilbert(ix[i], iy[i], iz[i]);
+}
+
+void orc_hilbert_decode(int n, const uint64_t* keys, unsigned* ixyz)
+{
+    for (int i = 0; i < n; ++i) orc::decodeHilbert(keys[i], ixyz + 3 * (size_t)i);
+}
+
+// keys for particles in a given box: include/tree.cuh:89-105
+void orc_keys(int n, const double* x, const double* y, const double* z, const double* h,
+              const double* box6, int periodic, uint64_t* keys)
+{
+    const orc::Box b = orc::makeBox(box6, periodic);
+    for (int i = 0; i < n; ++i) keys[i] = (h[i] > 0.0) ? orc::hilbert3D(x[i], y[i], z[i], b) : orc::kDeadKey;
+}
+
+void orc_open_box(int n, const double* x, const double* y, const double* z, double* box6)
+{
+    const orc::Box b = orc::openBox(n, x, y, z);
+    for (int a = 0; a < 3; ++a) { box6[2 * a] = b.lo[a]; box6[2 * a + 1] = b.hi[a]; }
+}
+
+void orc_node_geometry(int n, const uint64_t* start, const int* level, const double* box6,
+                       int periodic, double* c3, double* s3)
+{
+    const orc::Box b = orc::makeBox(box6, periodic);
+    for (int i = 0; i < n; ++i) orc::nodeGeometry(start[i], level[i], b, c3 + 3 * (size_t)i, s3 + 3 * (size_t)i);
+}
+
+void* orc_new(int quintic, int bucket)
+{
+    State* s = new State();
+    s->k = orc::kernelSpec(quintic);
+    s->bucket = bucket > 0 ? bucket : 64;
+    return s;
+}
+
+void orc_free(void* p) { delete static_cast<State*>(p); }
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_threads(int t)
+{
+#ifdef _OPENMP
+    if (t > 0) omp_set_num_threads(t);
+#else
+    (void)t;
+#endif
+}
+
+// Slot-like setters in the caller's (host) order, before build.
+void orc_set_pos(void* p, int n, const double* x, const double* y, const double* z, const double* h)
+{
+    State& s = *static_cast<State*>(p);
+    s.n = n;
+    s.x.assign(x, x + n); s.y.assign(y, y + n); s.z.assign(z, z + n); s.h.assign(h, h + n);
+    s.vx.clear(); s.vy.clear(); s.vz.clear(); s.ax.clear(); s.ay.clear(); s.az.clear();
+    s.haveOrder = 0;
+}
+
+void orc_set_vel(void* p, const double* vx, const double* vy, const double* vz)
+{
+    State& s = *static_cast<State*>(p);
+    s.vx.assign(vx, vx + s.n); s.vy.assign(vy, vy + s.n); s.vz.assign(vz, vz + s.n);
+}
+
+void orc_set_acc(void* p, const double* ax, const double* ay, const double* az)
+{
+    State& s = *static_cast<State*>(p);
+    s.ax.assign(ax, ax + s.n); s.ay.assign(ay, ay + s.n); s.az.assign(az, az + s.n);
+}
+
+// THERMO in host order, gathered into Hilbert order (as cosmo_upload_sorted)
+void orc_set_thermo_sorted(void* p, const double* pro2, const double* cs, const double* alpha, const double* u)
+{
+    State& s = *static_cast<State*>(p);
+    const int n = s.n;
+    s.pro2.resize(n); s.cs.resize(n); s.alpha.resize(n); s.u.resize(n);
+    for (int i = 0; i < n; ++i)
+    {
+        const int o = s.order[i];
+        s.pro2[i] = pro2[o]; s.cs[i] = cs[o]; s.alpha[i] = alpha[o]; s.u[i] = u[o];
+    }
+}
+
+void orc_set_h_sorted(void* p, const double* h)
+{
+    State& s = *static_cast<State*>(p);
+    s.h.assign(h, h + s.n);
+}
+
+void orc_build(void* p, int periodic, const double* box6)
+{
+    orc::build(*static_cast<State*>(p), periodic, box6);
+}
+
+int orc_get_int(void* p, int what)
+{
+    const State& s = *static_cast<State*>(p);
+    switch (what)
+    {
+        case 0: return s.n;
+        case 1: return s.nAlive;
+        case 2: return orc::nLeaves(s);
+        case 3: return (int)s.internalStart.size();
+        case 4: return (int)s.jList.size();
+        case 5: return s.itersRun;
+        case 6: return s.unconverged;
+        case 7: return (int)s.nodes.size();
+        default: return -1;
+    }
+}
+
+void orc_get_box(void* p, double* box6)
+{
+    const State& s = *static_cast<State*>(p);
+    for (int a = 0; a < 3; ++a) { box6[2 * a] = s.box.lo[a]; box6[2 * a + 1] = s.box.hi[a]; }
+}
+
+// Copies a named array out.  Names: keys order csTree leafLevel layout particleLeaf
+// internalStart internalLevel hmaxLeaf hmaxNode jOffset jCount jList x y z h rho gradh
+// divv xi ddivvdt fx fy fz du vsig divvF vx vy vz activePerIter pairsPerIter candPerIter
+int orc_get(void* p, const char* name, void* dst)
+{
+    const State& s = *static_cast<State*>(p);
+    auto cp = [&](const auto& v) {
+        std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+        return (int)v.size();
+    };
+    const std::string nm(name);
+    if (nm == "keys") return cp(s.keys);
+    if (nm == "order") return cp(s.order);
+    if (nm == "csTree") return cp(s.leafKey);
+    if (nm == "leafLevel") return cp(s.leafLevel);
+    if (nm == "layout") return cp(s.layout);
+    if (nm == "particleLeaf") return cp(s.particleLeaf);
+    if (nm == "internalStart") return cp(s.internalStart);
+    if (nm == "internalLevel") return cp(s.internalLevel);
+    if (nm == "hmaxLeaf") return cp(s.hmaxLeaf);
+    if (nm == "hmaxNode") return cp(s.hmaxNode);
+    if (nm == "jOffset") return cp(s.jOffset);
+    if (nm == "jCount") return cp(s.jCount);
+    if (nm == "jList") return cp(s.jList);
+    if (nm == "x") return cp(s.x);
+    if (nm == "y") return cp(s.y);
+    if (nm == "z") return cp(s.z);
+    if (nm == "h") return cp(s.h);
+    if (nm == "vx") return cp(s.vx);
+    if (nm == "vy") return cp(s.vy);
+    if (nm == "vz") return cp(s.vz);
+    if (nm == "rho") return cp(s.rho);
+    if (nm == "gradh") return cp(s.gradh);
+    if (nm == "divv") return cp(s.divv);
+    if (nm == "xi") return cp(s.xi);
+    if (nm == "ddivvdt") return cp(s.ddivvdt);
+    if (nm == "fx") return cp(s.fx);
+    if (nm == "fy") return cp(s.fy);
+    if (nm == "fz") return cp(s.fz);
+    if (nm == "du") return cp(s.f4);
+    if (nm == "vsig") return cp(s.vsig);
+    if (nm == "divvF") return cp(s.divvF);
+    if (nm == "activePerIter") return cp(s.activePerIter);
+    if (nm == "pairsPerIter") return cp(s.pairsPerIter);
+    if (nm == "candPerIter") return cp(s.candPerIter);
+    return -1;
+}
+
+int orc_get_size(void* p, const char* name)
+{
+    const State& s = *static_cast<State*>(p);
+    const std::string nm(name);
+    if (nm == "csTree" || nm == "layout") return orc::nLeaves(s) + 1;
+    if (nm == "leafLevel" || nm == "hmaxLeaf" || nm == "jCount") return orc::nLeaves(s);
+    if (nm == "jOffset") return (int)s.jOffset.size();
+    if (nm == "jList") return (int)s.jList.size();
+    if (nm == "internalStart" || nm == "internalLevel") return (int)s.internalStart.size();
+    if (nm == "hmaxNode") return (int)s.hmaxNode.size();
+    if (nm == "activePerIter" || nm == "pairsPerIter" || nm == "candPerIter") return (int)s.activePerIter.size();
+    return s.n;
+}
+
+void orc_hmax_all(void* p)
+{
+    State& s = *static_cast<State*>(p);
+    std::vector<int> all(orc::nLeaves(s));
+    std::iota(all.begin(), all.end(), 0);
+    orc::hmaxLeaves(s, all);
+}
+
+void orc_build_lists(void* p, int symmetric) { orc::buildLists(*static_cast<State*>(p), symmetric != 0); }
+
+int orc_density_solve(void* p, double pmass, double tolh, double hfact, int maxIter, int mode,
+                      int withGradients)
+{
+    return orc::densitySolve(*static_cast<State*>(p), pmass, tolh, hfact, maxIter, mode, withGradients);
+}
+
+// rho and d(rho)/d(h) at the current h of every live particle, over fresh gather lists
+void orc_closure(void* p, double pmass)
+{
+    State& s = *static_cast<State*>(p);
+    orc_hmax_all(p);
+    orc::buildLists(s, false);
+    if ((int)s.rho.size() != s.n) { s.rho.assign(s.n, 0.0); s.gradh.assign(s.n, 0.0); }
+    orc::closure(s, pmass);
+}
+
+void orc_force(void* p, double pmass, double hfact, double beta, double alphau, int disc, int pdv, int shock)
+{
+    State& s = *static_cast<State*>(p);
+    orc::buildLists(s, true);
+    orc::force(s, pmass, hfact, beta, alphau, disc, pdv, shock);
+}
+
+// Brute-force O(N^2) density sums at the given h (independent of tree and lists).
+// Positions and h in any order; open boundaries only.
+void orc_brute_density(int quintic, int n, const double* x, const double* y, const double* z,
+                       const double* h, double pmass, double* rho, double* gradh)
+{
+    const orc::KernelSpec k = orc::kernelSpec(quintic);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n; ++i)
+    {
+        double w = 0.0, g = 0.0;
+        if (!(h[i] > 0.0)) { rho[i] = 0.0; gradh[i] = 0.0; continue; }
+        const double hs = 1.0 / (h[i] * h[i]);
+        for (int j = 0; j < n; ++j)
+        {
+            if (!(h[j] > 0.0)) continue;
+            const double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+            const double q2 = (dx * dx + dy * dy + dz * dz) * hs;
+            if (q2 < k.rad2)
+            {
+                const double q = std::sqrt(q2);
+                double ww, dw;
+                orc::kern(k, q2, q, ww, dw);
+                w += ww;
+                g += -q * dw - 3.0 * ww;
+            }
+        }
+        const double hi1 = 1.0 / h[i], hi31 = hi1 * hi1 * hi1;
+        rho[i]   = w * k.cnormk * pmass * hi31;
+        gradh[i] = g * k.cnormk * pmass * hi31 * hi1;
+    }
+}
+
+} // extern "C"

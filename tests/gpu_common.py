@@ -1,0 +1,70 @@
+"""Shared helpers of the GPU parity tests: run one set through the library (C ABI) and the
+FP64 oracle on identical inputs, and compare stage by stage."""
+import numpy as np
+
+from cosmosphere_amd import Library, workloads
+from oracle import oracle as O
+
+
+def dataset(name):
+    """Small versions of the benchmark inputs plus edge cases."""
+    if name == "lattice":
+        return workloads.config2(npartx=12)
+    if name == "C1small":
+        c = workloads.config1(npartx=20)
+        return c
+    if name == "plummer_cubic":
+        c = workloads.config3(n=20000, seed=11)
+        c.update(kernel="cubic", quintic=False, hfact=1.2)
+        c["h"] = np.full(c["n"], 1.2 * c["n"] ** (-1.0 / 3.0))
+        return c
+    if name == "plummer_quintic":
+        return workloads.config3(n=20000, seed=12)
+    if name == "dead":
+        c = workloads.config2(npartx=10)
+        h = c["h"].copy()
+        h[::17] = -h[::17]          # dead particles (h <= 0) are parked after the live ones
+        c["h"] = h
+        return c
+    if name == "duplicates":
+        c = workloads.config2(npartx=10)
+        x, y, z = c["x"].copy(), c["y"].copy(), c["z"].copy()
+        x[:100] = 0.5; y[:100] = 0.5; z[:100] = 0.5   # > 64 keys in one level-21 cell
+        c.update(x=x, y=y, z=z)
+        return c
+    if name == "tiny":
+        c = workloads.config2(npartx=4)
+        return c
+    raise KeyError(name)
+
+
+def gpu_run(c, precision=0, gradients=True, mode=0, max_iter=100, v=None, a=None):
+    lib = Library(c["kernel"])
+    lib.set_option("precision", precision)
+    lib.set_option("max_iter", max_iter)
+    n = c["n"]
+    lib.arrays_init(n)
+    lib.upload(1, np.stack([c["x"], c["y"], c["z"]]))
+    lib.upload(2, c["h"])
+    lib.upload(3, np.zeros((3, n)) if v is None else v)
+    lib.upload(4, np.zeros((3, n)) if a is None else a)
+    iters = lib.solve_dens_h(n, c["pmass"], mode=mode, with_gradients=gradients, tolh=1e-4, hfact=c["hfact"])
+    return lib, iters
+
+
+def oracle_run(c, gradients=True, mode=0, max_iter=100, v=None, a=None):
+    n = c["n"]
+    o = O.Oracle(quintic=c["quintic"])
+    o.set_particles(c["x"], c["y"], c["z"], c["h"], v=np.zeros((3, n)) if v is None else v,
+                    a=np.zeros((3, n)) if a is None else a)
+    o.build()
+    unconv = o.density_solve(c["pmass"], 1e-4, c["hfact"], max_iter, mode, gradients)
+    return o, unconv
+
+
+def rel_err(a, b, floor=0.0):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.maximum(np.abs(b), floor)
+    den = np.where(den == 0, 1.0, den)
+    return float(np.max(np.abs(a - b) / den)) if a.size else 0.0

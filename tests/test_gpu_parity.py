@@ -1,0 +1,130 @@
+"""GPU parity: every stage of the hot path through the C ABI against the FP64 oracle.
+
+Bit-exact: keys, sort permutation, live count, leaf tree (csTree, levels, layout),
+internal nodes, node geometry, particle -> leaf map, j-leaf lists.
+Plain FP64 density solve: h, rho, d(rho)/d(h) within 1e-10 relative (summation order and
+fused multiply-add are the only differences), same iteration count.
+Mixed FP32-offset form: h within 1e-6 relative of the FP64 oracle, rho(h) closure within 1e-6.
+"""
+import numpy as np
+import pytest
+
+from gpu_common import dataset, gpu_run, oracle_run, rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+STAGE_SETS = ["lattice", "plummer_cubic", "plummer_quintic", "dead", "duplicates", "tiny"]
+
+
+@pytest.mark.parametrize("name", STAGE_SETS)
+def test_tree_stages_bit_exact(name):
+    c = dataset(name)
+    print("case", name, "n", c["n"], flush=True)
+    lib, _ = gpu_run(c, gradients=False)
+    o, _ = oracle_run(c, gradients=False)
+    na = o.geti("nAlive")
+    assert int(lib.stat("n_alive")) == na
+    np.testing.assert_array_equal(lib.array("keys"), o.get("keys"))
+    np.testing.assert_array_equal(lib.array("order"), o.get("order"))
+    L = o.geti("nLeaves")
+    assert int(lib.stat("n_leaves")) == L
+    cs = o.get("csTree")
+    np.testing.assert_array_equal(lib.array("leafStart"), cs[:-1])
+    np.testing.assert_array_equal(lib.array("leafLevel"), o.get("leafLevel"))
+    np.testing.assert_array_equal(lib.array("layout"), o.get("layout"))
+    np.testing.assert_array_equal(lib.array("particleLeaf"), o.get("particleLeaf")[:na])
+    np.testing.assert_array_equal(lib.array("intStart"), o.get("internalStart"))
+    np.testing.assert_array_equal(lib.array("intLevel"), o.get("internalLevel"))
+    # leaf geometry (FP64, bit-exact)
+    leaf_node = lib.array("leafNode")
+    box = o.box()
+    oc, osz = O.node_geometry(cs[:-1], o.get("leafLevel"), box)
+    for k, arr in enumerate(("nodeCx", "nodeCy", "nodeCz")):
+        np.testing.assert_array_equal(lib.array(arr)[leaf_node], oc[:, k])
+    for k, arr in enumerate(("nodeSx", "nodeSy", "nodeSz")):
+        np.testing.assert_array_equal(lib.array(arr)[leaf_node], osz[:, k])
+    # sorted positions
+    for f in ("x", "y", "z"):
+        np.testing.assert_array_equal(lib.array(f), o.get(f))
+
+
+@pytest.mark.parametrize("name", ["lattice", "C1small", "plummer_cubic", "plummer_quintic", "dead"])
+def test_density_plain_fp64(name):
+    c = dataset(name)
+    print("case", name, "n", c["n"], flush=True)
+    lib, iters = gpu_run(c, precision=0, gradients=False, max_iter=c.get("max_iter", 100))
+    o, unconv = oracle_run(c, gradients=False, max_iter=c.get("max_iter", 100))
+    assert unconv == 0
+    assert iters == o.geti("itersRun")
+    na = o.geti("nAlive")
+    h, h_ref = lib.array("h")[:na], o.get("h")[:na]
+    assert rel_err(h, h_ref) < 1e-10
+    assert rel_err(lib.array("rho")[:na], o.get("rho")[:na]) < 1e-10
+    g_ref = o.get("gradh")[:na]
+    assert rel_err(lib.array("gradh")[:na], g_ref, floor=np.abs(g_ref).max() * 1e-6) < 1e-9
+    # accepted pairs: the oracle counts the same pairs
+    assert abs(lib.stat("pairs") - sum(o.get("pairsPerIter"))) <= 1e-6 * sum(o.get("pairsPerIter")) + 2
+
+
+@pytest.mark.parametrize("name", ["lattice", "plummer_quintic"])
+def test_jlists_bit_exact(name):
+    """All-leaf gather lists at the converged h (left by the gradient sweep)."""
+    c = dataset(name)
+    lib, _ = gpu_run(c, precision=0, gradients=True)
+    o, _ = oracle_run(c, gradients=False)
+    o.set_h_sorted(lib.array("h"))
+    o.hmax_all()
+    o.build_lists(symmetric=False)
+    np.testing.assert_array_equal(lib.array("hmaxLeaf"), o.get("hmaxLeaf"))
+    np.testing.assert_array_equal(lib.array("jCount"), o.get("jCount"))
+    np.testing.assert_array_equal(lib.array("jOffset"), o.get("jOffset"))
+    np.testing.assert_array_equal(lib.array("jList"), o.get("jList"))
+
+
+@pytest.mark.parametrize("name", ["lattice", "plummer_quintic"])
+def test_gradient_sweep(name):
+    c = dataset(name)
+    n = c["n"]
+    idx = np.arange(n)
+    from cosmosphere_amd import workloads
+    v = np.stack([0.3 * c["x"] - 0.1 * c["y"] + 0.01 * workloads.gaussian(7, 0, idx),
+                  0.2 * c["z"] + 0.05 * c["x"], -0.4 * c["y"] + 0.01 * workloads.gaussian(7, 1, idx)])
+    a = np.stack([np.sin(3 * c["x"]), np.cos(2 * c["y"]), c["z"] ** 2])
+    lib = None
+    from cosmosphere_amd import Library
+    lib = Library(c["kernel"])
+    lib.set_option("precision", 0)
+    lib.set_option("max_iter", 100)
+    out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], tolh=1e-4, hfact=c["hfact"], v=v, a=a)
+    o, _ = oracle_run(c, gradients=True, v=v, a=a)
+    order = o.get("order")
+
+    def host(name_):
+        arr = np.empty(n)
+        arr[order] = o.get(name_)
+        return arr
+    assert rel_err(out["h"], host("h")) < 1e-10
+    assert rel_err(out["rho"], host("rho")) < 1e-10
+    for f in ("divv", "ddivvdt"):
+        ref = host(f)
+        assert rel_err(out[f], ref, floor=np.abs(ref).max() * 1e-6) < 1e-7, f
+    assert np.max(np.abs(out["xi"] - host("xi"))) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["lattice", "C1small", "plummer_cubic", "plummer_quintic"])
+def test_density_mixed_vs_fp64_oracle(name):
+    c = dataset(name)
+    print("case", name, "n", c["n"], flush=True)
+    lib, iters = gpu_run(c, precision=1, gradients=False, max_iter=c.get("max_iter", 100))
+    o, unconv = oracle_run(c, gradients=False, max_iter=c.get("max_iter", 100))
+    na = o.geti("nAlive")
+    h = lib.array("h")[:na]
+    h_ref = o.get("h")[:na]
+    eh = rel_err(h, h_ref)
+    # the closure evaluates rho(h) at the final h; compare with the oracle's closure at its h
+    o.closure(c["pmass"])
+    er = rel_err(lib.array("rho")[:na], o.get("rho")[:na])
+    print("mixed", name, "iters", iters, o.geti("itersRun"), "max rel err h %.3e rho %.3e" % (eh, er))
+    assert eh < 1e-6
+    assert er < 1e-6
