@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark: particle h-solves/s and pair-interactions/s of the smoothing-length solve on
+one MI355X, mixed precision form vs plain FP64, plus the FP64 CPU oracle on a bounded
+sample of the same workload.
+
+A step is one complete solve of the workload from the same device-resident inputs:
+restore positions and h (device-to-device), then bounding box, Hilbert keys, sort,
+permutation, leaf tree, node geometry, and the Newton-Raphson loop (hmax, neighbour-leaf
+lists, density kernel, compaction) until every particle has converged -- the reference's
+solveDensH (src/density_base.cu:720-1015) in FLAT_PARTICLE mode, density only.  The mixed
+form also evaluates rho(h) at the converged h (its closure), so its outputs are consistent.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+
+  python bench.py [--steps K] [--warmup W] [--config C3] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from cosmosphere_amd import Library, workloads  # noqa: E402
+from cosmosphere_amd.api import ACCEL, HSML, POS, VEL  # noqa: E402
+
+METRIC = "particle h-solves/s + pair-interactions/s, mixed vs FP64, 1 MI355X"
+FP32_PEAK = 157.3   # TFLOP/s, MI355X vector FP32
+FP64_PEAK = 78.6    # TFLOP/s, MI355X vector FP64
+# Flop convention of the density pair sum (fixed): 9 per candidate pair (3 sub, r^2, scale),
+# 21 more per accepted pair (sqrt, kernel polynomial, accumulation).
+FLOPS_CAND, FLOPS_ACC = 9.0, 21.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_config(name):
+    if name == "C1":
+        return workloads.config1()
+    if name == "C2":
+        return workloads.config2()
+    if name == "C3":
+        return workloads.config3()
+    if name.startswith("C3:"):
+        return workloads.config3(n=int(name[3:]))
+    raise SystemExit("unknown config " + name)
+
+
+def run_precision(lib, c, prec, steps, warmup):
+    n = c["n"]
+    lib.set_option("precision", prec)
+    lib.set_option("max_iter", c.get("max_iter", 100))
+    lib.set_option("closure", -1)
+    lib.set_option("tree_only", 0)
+
+    def one():
+        lib.restore_inputs()
+        return lib.solve_dens_h(n, c["pmass"], 0, False, None, 1e-4, c["hfact"])
+
+    for w in range(warmup):
+        t = time.perf_counter()
+        it = one()
+        log("  warmup %d: %d iterations, %.1f ms" % (w, it, 1e3 * (time.perf_counter() - t)))
+    keys = ["nrkern", "launches", "pairs", "candidates", "closure", "closure_pairs", "closure_candidates",
+            "jbuild", "compact", "total", "iters", "bbox", "keys", "sort", "permute", "tree", "nodes"]
+    acc = {k: 0.0 for k in keys}
+    lib.sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        it = one()
+        for key in keys:
+            acc[key] += lib.stat(key)
+        log("  step %d: %d iterations" % (k, it))
+    lib.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    alive = lib.stat("n_alive")
+    flops = FLOPS_CAND * acc["candidates"] + FLOPS_ACC * acc["pairs"]
+    res = {
+        "wall_s": wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "h_solves_per_s": alive * steps / wall,
+        "pairs_per_s_kernel": acc["pairs"] / acc["nrkern"] if acc["nrkern"] > 0 else 0.0,
+        "pairs_per_s_solve": acc["pairs"] / wall,
+        "pairs_per_step": acc["pairs"] / steps,
+        "candidates_per_step": acc["candidates"] / steps,
+        "iterations": acc["iters"] / steps,
+        "kernel_launches_per_step": acc["launches"] / steps,
+        "kernel_avg_ms": 1e3 * acc["nrkern"] / max(acc["launches"], 1),
+        "kernel_tflops": flops / acc["nrkern"] / 1e12 if acc["nrkern"] > 0 else 0.0,
+        "phase_ms": {k: 1e3 * acc[k] / steps for k in ("bbox", "keys", "sort", "permute", "tree", "nodes", "jbuild",
+                                                          "nrkern", "compact", "closure", "total")},
+        "n_leaves": lib.stat("n_leaves"),
+    }
+    res["h_sorted"] = lib.array("h")
+    return res
+
+
+def cpu_baseline(c, gpu_h_fp64, gpu_h_mixed, seconds):
+    """FP64 oracle on a bounded sample: all particles of randomly chosen leaves, solved over
+    the full tree (in FLAT mode a particle's h depends on positions only, so the sample's h
+    equal the full solve's).  Rate = sampled particles / (sample NR time + the tree build's
+    share for the sample)."""
+    from oracle import oracle as O
+    o = O.Oracle(quintic=c["quintic"])
+    o.set_particles(c["x"], c["y"], c["z"], c["h"])
+    t = time.perf_counter()
+    o.build()
+    t_tree = time.perf_counter() - t
+    na = o.geti("nAlive")
+    pl = o.get("particleLeaf")[:na]
+    rng = np.random.default_rng(0)
+
+    def sample(frac):
+        pick = rng.choice(na, size=max(1, int(frac * na)), replace=False)
+        return np.unique(pl[pick])
+
+    frac = 2e-4
+    leaves = sample(frac)
+    t = time.perf_counter()
+    o.density_solve_leaves(c["pmass"], leaves, 1e-4, c["hfact"], c.get("max_iter", 100))
+    t_nr = time.perf_counter() - t
+    if t_nr < 0.3 * seconds:
+        frac = min(1.0, frac * seconds / max(t_nr, 1e-3))
+        leaves = sample(frac)
+        t = time.perf_counter()
+        o.density_solve_leaves(c["pmass"], leaves, 1e-4, c["hfact"], c.get("max_iter", 100))
+        t_nr = time.perf_counter() - t
+    lay = o.get("layout")
+    idx = np.concatenate([np.arange(lay[l], lay[l + 1]) for l in leaves])
+    npart = idx.size
+    rate = npart / (t_nr + t_tree * npart / na)
+    h_ref = o.get("h")[idx]
+    err = {"max_rel_err_h_fp64": float(np.max(np.abs(gpu_h_fp64[idx] - h_ref) / h_ref)) if gpu_h_fp64 is not None else None,
+           "max_rel_err_h_mixed": float(np.max(np.abs(gpu_h_mixed[idx] - h_ref) / h_ref)) if gpu_h_mixed is not None else None,
+           "p99_rel_err_h_mixed": float(np.percentile(np.abs(gpu_h_mixed[idx] - h_ref) / h_ref, 99))
+           if gpu_h_mixed is not None else None,
+           "particles": int(npart)}
+    base = {"value": rate, "unit": "particle h-solves/s", "cores": O.num_threads(), "kind": "port",
+            "sample": "FP64 oracle (oracle/sph_oracle.cpp, OpenMP) on all %d particles of %d randomly chosen leaves "
+                      "(%.3f%% of the live particles), solved to convergence over the full %d-particle tree; tree "
+                      "build %.2f s charged pro rata, NR %.2f s" % (npart, leaves.size, 100.0 * npart / na, na, t_tree,
+                                                                     t_nr)}
+    return base, err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C3")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this solver runs a particle set (or an ensemble of sets) on one MI355X; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    t = time.perf_counter()
+    c = load_config(args.config)
+    log("workload %s: %d particles, %s kernel (generated in %.1f s)" % (c["name"], c["n"], c["kernel"],
+                                                                        time.perf_counter() - t))
+    lib = Library(c["kernel"])
+    n = c["n"]
+    lib.arrays_init(n)
+    lib.upload(POS, np.stack([c["x"], c["y"], c["z"]]))
+    lib.upload(HSML, c["h"])
+    lib.upload(VEL, np.zeros((3, n)))
+    lib.upload(ACCEL, np.zeros((3, n)))
+    lib.stash_inputs()
+
+    res = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        if args.only not in ("both", name):
+            continue
+        log("precision %s" % name)
+        res[name] = run_precision(lib, c, prec, args.steps, args.warmup)
+        r = res[name]
+        log("  %s: %.1f ms/step, %.3e h-solves/s, %.3e pairs/s (kernel), kernel %.2f TFLOP/s, iters %.1f"
+            % (name, r["ms_per_step"], r["h_solves_per_s"], r["pairs_per_s_kernel"], r["kernel_tflops"],
+               r["iterations"]))
+
+    head = res.get("mixed") or res.get("fp64")
+    head_name = "mixed" if "mixed" in res else "fp64"
+    peak = FP32_PEAK if head_name == "mixed" else FP64_PEAK
+    out = {
+        "metric": METRIC,
+        "value": head["h_solves_per_s"],
+        "unit": "particle h-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair sums + fp64 Newton" if head_name == "mixed" else "fp64",
+        "data": "synthetic",
+        "config": {"workload": c["name"], "n": n, "kernel": c["kernel"], "hfact": c["hfact"], "tolh": 1e-4,
+                   "max_iter": c.get("max_iter", 100), "mode": "FLAT_PARTICLE density-only solve",
+                   "parallelism": "single GPU"},
+        "pair_interactions_per_s": head["pairs_per_s_kernel"],
+        "pair_interactions_per_s_solve": head["pairs_per_s_solve"],
+        "pairs_per_step": head["pairs_per_step"],
+        "candidates_per_step": head["candidates_per_step"],
+        "iterations": head["iterations"],
+        "phase_ms": head["phase_ms"],
+        "roofline": {"bound": "valu", "achieved": head["kernel_tflops"], "peak": peak, "unit": "TFLOP/s",
+                     "frac": head["kernel_tflops"] / peak, "traffic": None,
+                     "kernel": "densityKernel (%s NR pair sums)" % head_name,
+                     "kernel_avg_ms": head["kernel_avg_ms"],
+                     "flops_convention": "9 per candidate + 21 per accepted pair"},
+    }
+    if "fp64" in res and "mixed" in res:
+        f = res["fp64"]
+        out["fp64"] = {"value": f["h_solves_per_s"], "ms_per_step": f["ms_per_step"],
+                       "pair_interactions_per_s": f["pairs_per_s_kernel"], "kernel_tflops": f["kernel_tflops"],
+                       "roofline_frac": f["kernel_tflops"] / FP64_PEAK, "iterations": f["iterations"],
+                       "phase_ms": f["phase_ms"]}
+        out["mixed_vs_fp64"] = res["mixed"]["h_solves_per_s"] / f["h_solves_per_s"]
+        hm, hf = res["mixed"]["h_sorted"], f["h_sorted"]
+        na = int(lib.stat("n_alive"))
+        out["mixed_vs_fp64_h_max_rel"] = float(np.max(np.abs(hm[:na] - hf[:na]) / hf[:na]))
+    if not args.no_cpu:
+        log("cpu baseline (oracle sample)")
+        base, err = cpu_baseline(c, res.get("fp64", {}).get("h_sorted"), res.get("mixed", {}).get("h_sorted"),
+                                 args.cpu_seconds)
+        out["cpu_baseline"] = base
+        out["parity_sample"] = err
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

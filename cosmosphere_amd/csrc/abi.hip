@@ -325,6 +325,7 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "max_iter") s.maxIter = (int)value;
     else if (nm == "mapping") s.mapping = (int)value;
     else if (nm == "closure") s.closureOn = (int)value;
+    else if (nm == "tree_only") s.treeOnly = (int)value;
     else
     {
         std::fprintf(stderr, "FATAL: cosmo_ext_set_option: unknown option '%s'\n", name);

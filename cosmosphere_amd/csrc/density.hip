@@ -30,6 +30,8 @@
 #include "prims.hpp"
 #include "tile.hpp"
 
+#include <type_traits>
+
 namespace csph
 {
 
@@ -88,11 +90,13 @@ __device__ inline void blockCounters(unsigned long long* __restrict__ counters, 
 template<class Real, bool PER, int MODE>
 __global__ __launch_bounds__(kDensThreads) void densityKernel(DensArgs A)
 {
-    __shared__ int  segStart[kDensWaves][kWave];
-    __shared__ int  segJ[kDensWaves][kWave];
-    __shared__ Real tx[kDensWaves][kWave], ty[kDensWaves][kWave], tz[kDensWaves][kWave];
     constexpr bool kMixed = sizeof(Real) == 4;
-    const int w = threadIdx.x >> 6, lane = laneId();
+    // mixed: j offsets as 32-bit fixed point from the i-leaf centre; plain: FP64 positions
+    using TileT = typename std::conditional<kMixed, int, double>::type;
+    __shared__ int   segStart[kDensWaves][kWave];
+    __shared__ int   segJ[kDensWaves][kWave];
+    __shared__ TileT tx[kDensWaves][kWave], ty[kDensWaves][kWave], tz[kDensWaves][kWave];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = laneId();
     const int a = blockIdx.x * kDensWaves + w;
     unsigned long long nAcc = 0, nCand = 0;
 
@@ -103,12 +107,27 @@ __global__ __launch_bounds__(kDensThreads) void densityKernel(DensArgs A)
         const uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
         DBox box{};
         if constexpr (PER) box = A.boxes[A.leafSet[iLeaf]];
-        double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+        // Mixed form: offsets from the i-leaf centre c in units of range / 2^29, where range
+        // = leaf half-size + R hmax bounds every i offset.  The difference of two such
+        // integers is exact, so a pair separation is rounded once, to FP32, relative to
+        // itself (not to the offsets).  j offsets are clamped to +-2^30: a clamped j is at
+        // least `range` from every i, beyond R h_i, and differences stay inside int32.
+        double c0 = 0.0, c1 = 0.0, c2 = 0.0, S = 0.0, invS2 = 0.0;
         if constexpr (kMixed)
         {
             const int node = A.leafNode[iLeaf];
             c0 = A.cx[node]; c1 = A.cy[node]; c2 = A.cz[node];
+            double hm = A.hmaxLeaf[iLeaf];
+            if (!isfinite(hm) || hm <= 0.0) hm = 0.0;
+            const double range = fmax(fmax(A.sx[node], A.sy[node]), A.sz[node]) + kRad * hm;
+            S = 536870912.0 / range;   // 2^29
+            invS2 = 1.0 / (S * S);
         }
+        auto toFixed = [&](double v) -> int {
+            v = rint(v * S);
+            v = fmin(fmax(v, -1073741824.0), 1073741824.0);   // +-2^30
+            return (int)v;
+        };
         for (uint32_t q0 = p0; q0 < p1; q0 += kWave)
         {
             const uint32_t i     = q0 + lane;
@@ -119,12 +138,18 @@ __global__ __launch_bounds__(kDensThreads) void densityKernel(DensArgs A)
             const double xi = valid ? A.x[i] : 0.0, yi = valid ? A.y[i] : 0.0, zi = valid ? A.z[i] : 0.0;
             double hcur = valid ? A.h[i] : 1.0;
             const double hsi = 1.0 / (hcur * hcur);
-            const Real xr = kMixed ? Real(xi - c0) : Real(xi);
-            const Real yr = kMixed ? Real(yi - c1) : Real(yi);
-            const Real zr = kMixed ? Real(zi - c2) : Real(zi);
-            const Real hsr = Real(hsi);
-            // plain: direct FP64 sums; mixed: per-tile FP32 sums, Kahan-compensated
-            Real sw = 0, sg = 0, cw = 0, cg = 0;
+            TileT xr, yr, zr;
+            if constexpr (kMixed) { xr = toFixed(xi - c0); yr = toFixed(yi - c1); zr = toFixed(zi - c2); }
+            else                  { xr = xi; yr = yi; zr = zi; }
+            // q^2 = r^2 h^-2 in fixed-point units: hi part for the candidate test, the
+            // remainder refines it for accepted pairs
+            const double hsd = kMixed ? hsi * invS2 : hsi;
+            const Real hsr = Real(hsd);
+            const Real hsrLo = kMixed ? Real(hsd - (double)hsr) : Real(0);
+            // Sums are FP64 in both forms: the mixed form evaluates each pair term in FP32
+            // and adds it to an FP64 accumulator (full rate on this GPU, cheaper than FP32
+            // compensated summation and exact to the rounding of the terms themselves).
+            double sw = 0, sg = 0;
             int  acc = 0;
             uint32_t cand = 0;
             auto load = [&](int l, int j) {
@@ -138,54 +163,59 @@ __global__ __launch_bounds__(kDensThreads) void densityKernel(DensArgs A)
                         if (fabs(yj) > 0.5 * box.l[1]) yj -= box.l[1] * copysign(1.0, yj);
                         if (fabs(zj) > 0.5 * box.l[2]) zj -= box.l[2] * copysign(1.0, zj);
                     }
+                    tx[w][l] = toFixed(xj); ty[w][l] = toFixed(yj); tz[w][l] = toFixed(zj);
                 }
-                tx[w][l] = Real(xj); ty[w][l] = Real(yj); tz[w][l] = Real(zj);
+                else
+                {
+                    tx[w][l] = xj; ty[w][l] = yj; tz[w][l] = zj;
+                }
             };
             auto consume = [&](int nt) {
-                Real tw = 0, tg = 0;
+#pragma unroll 4
                 for (int k = 0; k < nt; ++k)
                 {
-                    Real dx = xr - tx[w][k], dy = yr - ty[w][k], dz = zr - tz[w][k];
-                    if constexpr (PER && !kMixed)
+                    Real dx, dy, dz;
+                    if constexpr (kMixed)
                     {
-                        if (fabs(dx) > 0.5 * box.l[0]) dx = dx - box.l[0] * copysign(1.0, dx);
-                        if (fabs(dy) > 0.5 * box.l[1]) dy = dy - box.l[1] * copysign(1.0, dy);
-                        if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
+                        dx = (Real)(xr - tx[w][k]); dy = (Real)(yr - ty[w][k]); dz = (Real)(zr - tz[w][k]);
                     }
-                    const Real q2 = (dx * dx + dy * dy + dz * dz) * hsr;
+                    else
+                    {
+                        dx = xr - tx[w][k]; dy = yr - ty[w][k]; dz = zr - tz[w][k];
+                        if constexpr (PER)
+                        {
+                            if (fabs(dx) > 0.5 * box.l[0]) dx = dx - box.l[0] * copysign(1.0, dx);
+                            if (fabs(dy) > 0.5 * box.l[1]) dy = dy - box.l[1] * copysign(1.0, dy);
+                            if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
+                        }
+                    }
+                    const Real r2 = dx * dx + dy * dy + dz * dz;
+                    Real q2 = r2 * hsr;
                     if (q2 < Real(kRad2))
                     {
                         Real q;
-                        if constexpr (kMixed) q = __builtin_amdgcn_sqrtf(q2);
-                        else                  q = sqrt(q2);
-                        Real wk, dwk;
-                        kernWdW<Real>(q2, q, wk, dwk);
                         if constexpr (kMixed)
                         {
-                            tw += wk;
-                            tg += -q * dwk - Real(3) * wk;
+                            q2 = fmaf(r2, hsrLo, q2);
+                            q = __builtin_amdgcn_sqrtf(q2);
                         }
                         else
                         {
-                            sw += wk;                       // list order, as the reference
-                            sg += -q * dwk - Real(3) * wk;
+                            q = sqrt(q2);
                         }
+                        Real wk, dwk;
+                        kernWdW<Real>(q2, q, wk, dwk);
+                        sw += (double)wk;                               // list order, as the reference
+                        sg += (double)(-q * dwk - Real(3) * wk);
                         ++acc;
                     }
-                }
-                if constexpr (kMixed)
-                {
-                    Real y1 = tw - cw, t1 = sw + y1;
-                    cw = (t1 - sw) - y1; sw = t1;
-                    Real y2 = tg - cg, t2 = sg + y2;
-                    cg = (t2 - sg) - y2; sg = t2;
                 }
                 cand += (uint32_t)nt;
             };
             forEachTile(segStart[w], segJ[w], A.layout, A.jList, lo, hi, load, consume);
 
-            const double wsum = kMixed ? (double)sw - (double)cw : (double)sw;
-            const double gsum = kMixed ? (double)sg - (double)cg : (double)sg;
+            const double wsum = sw;
+            const double gsum = sg;
             if (doLane)
             {
                 nAcc += (unsigned long long)acc;

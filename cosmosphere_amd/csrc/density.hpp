@@ -19,6 +19,8 @@ struct DensArgs
     const int*      leafNode;
     const int*      leafSet;
     const double*   cx; const double* cy; const double* cz;
+    const double*   sx; const double* sy; const double* sz;
+    const double*   hmaxLeaf;
     const double*   x; const double* y; const double* z;
     double*         h;
     double*         rho;

@@ -107,6 +107,8 @@ static DensArgs densArgs(State& s, const int* leaves, int nLeaves, double pmass,
     a.leafNode = s.leafNode.as<int>();
     a.leafSet = s.leafSet.as<int>();
     a.cx = s.nodeCx.as<double>(); a.cy = s.nodeCy.as<double>(); a.cz = s.nodeCz.as<double>();
+    a.sx = s.nodeSx.as<double>(); a.sy = s.nodeSy.as<double>(); a.sz = s.nodeSz.as<double>();
+    a.hmaxLeaf = s.hmaxLeaf.as<double>();
     a.x = s.x.as<double>(); a.y = s.y.as<double>(); a.z = s.z.as<double>();
     a.h = s.h.as<double>();
     a.rho = s.rho.as<double>();
@@ -138,6 +140,7 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
     s.t = Timings{};
     buildTree(s, withGradients, periodicBox, sortScratch());
     t = s.t;
+    if (s.treeOnly) return t;
     const int n = s.n, na = s.totalAlive, L = s.nLeaves;
     const bool per = periodicOf(s);
 
@@ -213,7 +216,8 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         }
     }
 
-    // closure (mixed density-only form): rho and d(rho)/d(h) at the converged h
+    // closure (mixed density-only form): rho and d(rho)/d(h) at the converged h, with FP64
+    // sums, so the returned rho belongs to the returned h
     if (closure && !withGradients && L > 0 && na > 0)
     {
         Ev c0, c1;
@@ -221,7 +225,7 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         hmaxLeaves(s, nullptr, L);
         buildLists(s, nullptr, L, false);
         unsigned long long* ctr = zeroCounters(s);
-        launchDensity(densArgs(s, nullptr, L, pmass, tolh, hfact, ctr), s.precision, 2, per, st);
+        launchDensity(densArgs(s, nullptr, L, pmass, tolh, hfact, ctr), kPlainFP64, 2, per, st);
         c1.rec(st);
         unsigned long long c[3];
         readCounters(s, c);

@@ -137,6 +137,7 @@ struct State
     int      precision = kPlainFP64;
     int      mapping = kMapAuto;
     int      closureOn = -1;   // -1: on for mixed, off for plain
+    int      treeOnly = 0;     // verification: stop after the tree build
     double   uploadSeconds = 0.0, downloadSeconds = 0.0;
     Timings      t;
     ForceTimings ft;

@@ -31,7 +31,7 @@ __device__ inline void forEachTile(int* __restrict__ segStart, int* __restrict__
             jc = (int)layout[jl + 1] - js;
         }
         const int incl  = waveInclusiveScan(jc);
-        const int total = __shfl(incl, kWave - 1, kWave);
+        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));   // wave-uniform
         segStart[lane] = incl - jc;
         segJ[lane]     = js;
         const int nseg = (int)min((uint32_t)kWave, hi - e0);
@@ -51,7 +51,7 @@ __device__ inline void forEachTile(int* __restrict__ segStart, int* __restrict__
                 load(lane, segJ[a] + slot - segStart[a]);
             }
             waveLdsSync();
-            consume(min(kWave, total - s0));
+            consume(__builtin_amdgcn_readfirstlane(min(kWave, total - s0)));
             waveLdsSync();
         }
     }

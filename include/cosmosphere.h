@@ -91,7 +91,8 @@ int cosmo_ext_solve_dens_h(int n, double pmass, int mode, int withGradients, int
                            double tolh, double hfact);
 
 /* Options: "precision" (0 = plain FP64, 1 = mixed FP32-offset form), "max_iter",
- * "mapping" (0 auto, 1 wave per leaf, 2 flat), "closure" (-1 auto, 0 off, 1 on).
+ * "mapping" (0 auto, 1 wave per leaf, 2 flat), "closure" (-1 auto, 0 off, 1 on),
+ * "tree_only" (1: stop after the tree build, for verification of the tree stages).
  * The environment variable COSMO_PRECISION=mixed|fp64 sets the precision for
  * densityiterate_gpu_c. */
 void cosmo_ext_set_option(const char* name, double value);

@@ -61,6 +61,9 @@ def lib():
         L.orc_density_solve.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_density_solve.restype = ctypes.c_int
+        L.orc_density_solve_leaves.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_int, _i32p, ctypes.c_int]
+        L.orc_density_solve_leaves.restype = ctypes.c_int
         L.orc_closure.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_force.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -196,6 +199,12 @@ class Oracle:
     def density_solve(self, pmass, tolh=1e-4, hfact=-1.0, max_iter=100, mode=0, with_gradients=False):
         return lib().orc_density_solve(self._p, float(pmass), float(tolh), float(hfact),
                                        int(max_iter), int(mode), int(with_gradients))
+
+    def density_solve_leaves(self, pmass, leaves, tolh=1e-4, hfact=-1.0, max_iter=100):
+        """FLAT-mode solve of the particles of the given (ascending) leaves only."""
+        lv = np.ascontiguousarray(leaves, dtype=np.int32)
+        return lib().orc_density_solve_leaves(self._p, float(pmass), float(tolh), float(hfact), int(max_iter),
+                                              lv, lv.size)
 
     def closure(self, pmass):
         lib().orc_closure(self._p, float(pmass))

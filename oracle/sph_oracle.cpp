@@ -715,17 +715,31 @@ static void gradientParticle(State& s, int i, double pmass)
 // The solve: src/density_base.cu:720-1015.  mode 0 = FLAT_PARTICLE (only unconverged
 // particles are updated), mode 1 = WARP_PER_LEAF (every particle of an active leaf is
 // recomputed, src/density_base.cu:361-364).  Returns the number of unconverged particles.
+// subset (optional): solve only the particles of these leaves (ascending).  In FLAT mode a
+// particle's h sequence depends on positions alone, so the subset's results equal the full
+// solve's -- used for bounded CPU timing samples.
 int densitySolve(State& s, double pmass, double tolh, double hfact, int maxIter, int mode,
-                 int withGradients)
+                 int withGradients, const std::vector<int>* subset = nullptr)
 {
     if (!(hfact > 0.0)) hfact = s.k.hfactDefault;
     const int n = s.n, L = nLeaves(s);
     s.rho.assign(n, 0.0);
     s.gradh.assign(n, 0.0);
     std::vector<int> conv(n, 0);
-    std::vector<int> active(s.nAlive), leaves(L);
-    std::iota(active.begin(), active.end(), 0);
-    std::iota(leaves.begin(), leaves.end(), 0);
+    std::vector<int> active, leaves;
+    if (subset)
+    {
+        leaves = *subset;
+        for (int l : leaves)
+            for (unsigned i = s.layout[l]; i < s.layout[l + 1]; ++i) active.push_back((int)i);
+    }
+    else
+    {
+        active.resize(s.nAlive);
+        leaves.resize(L);
+        std::iota(active.begin(), active.end(), 0);
+        std::iota(leaves.begin(), leaves.end(), 0);
+    }
     s.itersRun = 0;
     s.activePerIter.clear();
     s.pairsPerIter.clear();
@@ -1164,6 +1178,13 @@ int orc_density_solve(void* p, double pmass, double tolh, double hfact, int maxI
                       int withGradients)
 {
     return orc::densitySolve(*static_cast<State*>(p), pmass, tolh, hfact, maxIter, mode, withGradients);
+}
+
+int orc_density_solve_leaves(void* p, double pmass, double tolh, double hfact, int maxIter, const int* leaves,
+                             int nLeaves)
+{
+    std::vector<int> sub(leaves, leaves + nLeaves);
+    return orc::densitySolve(*static_cast<State*>(p), pmass, tolh, hfact, maxIter, 0, 0, &sub);
 }
 
 // rho and d(rho)/d(h) at the current h of every live particle, over fresh gather lists

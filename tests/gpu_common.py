@@ -27,6 +27,7 @@ def dataset(name):
         c["h"] = h
         return c
     if name == "duplicates":
+        # tree stages only: > 64 coincident particles have no Newton fixed point
         c = workloads.config2(npartx=10)
         x, y, z = c["x"].copy(), c["y"].copy(), c["z"].copy()
         x[:100] = 0.5; y[:100] = 0.5; z[:100] = 0.5   # > 64 keys in one level-21 cell
@@ -38,10 +39,11 @@ def dataset(name):
     raise KeyError(name)
 
 
-def gpu_run(c, precision=0, gradients=True, mode=0, max_iter=100, v=None, a=None):
+def gpu_run(c, precision=0, gradients=True, mode=0, max_iter=100, v=None, a=None, tree_only=False):
     lib = Library(c["kernel"])
     lib.set_option("precision", precision)
     lib.set_option("max_iter", max_iter)
+    lib.set_option("tree_only", int(tree_only))
     n = c["n"]
     lib.arrays_init(n)
     lib.upload(1, np.stack([c["x"], c["y"], c["z"]]))

@@ -1,0 +1,161 @@
+"""GPU tests of the C-ABI contract (src/arrays_c_api.cu, src/dens_c_api.cu, src/force_c_api.cu):
+transfers gather / scatter through the sort permutation so the caller only sees its own
+order, repeated solves, resizing, the WARP_PER_LEAF mode, periodic boxes, and the abort
+paths that need a device (each in a subprocess)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from cosmosphere_amd import Library, workloads
+from cosmosphere_amd.api import ACCEL, DENS_OUT, HSML, POS, THERMO, VEL
+from gpu_common import gpu_run, oracle_run, rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_transfers_round_trip_in_caller_order():
+    c = workloads.config3(n=5000, seed=31)
+    c.update(kernel="cubic", quintic=False, hfact=1.2, h=np.full(5000, 1.2 * 5000 ** (-1 / 3)))
+    lib = Library("cubic")
+    lib.set_option("precision", 0)
+    n = c["n"]
+    rng = np.random.default_rng(0)
+    v = rng.normal(size=(3, n))
+    out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], hfact=1.2, v=v)
+    # positions come back exactly as sent, in the caller's order
+    np.testing.assert_array_equal(lib.download(POS, n), np.stack([c["x"], c["y"], c["z"]]))
+    np.testing.assert_array_equal(lib.download(VEL, n), v)
+    # upload_sorted then download is the identity in the caller's order
+    th = rng.normal(size=(4, n))
+    lib.upload_sorted(THERMO, th)
+    np.testing.assert_array_equal(lib.download(THERMO, n), th)
+    # the downloaded h is the oracle's h, in the caller's order
+    o, _ = oracle_run(c, gradients=True, v=v)
+    h_ref = np.empty(n)
+    h_ref[o.get("order")] = o.get("h")
+    assert rel_err(out["h"], h_ref) < 1e-10
+
+
+def test_repeated_solves_and_resize():
+    lib = Library("cubic")
+    lib.set_option("precision", 0)
+    for npartx, shift in ((8, 0.0), (8, 0.013), (11, 0.0)):
+        c = workloads.config2(npartx=npartx)
+        x = c["x"] + shift
+        out = lib.solve(x, c["y"], c["z"], c["h"], c["pmass"], hfact=1.2)
+        o, _ = oracle_run(dict(c, x=x), gradients=True)
+        h_ref = np.empty(c["n"])
+        h_ref[o.get("order")] = o.get("h")
+        assert rel_err(out["h"], h_ref) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["lattice", "plummer_quintic"])
+def test_warp_per_leaf_mode(name):
+    from gpu_common import dataset
+    c = dataset(name)
+    lib, iters = gpu_run(c, precision=0, gradients=False, mode=1)
+    o, unconv = oracle_run(c, gradients=False, mode=1)
+    assert unconv == 0 and iters == o.geti("itersRun")
+    na = o.geti("nAlive")
+    assert rel_err(lib.array("h")[:na], o.get("h")[:na]) < 1e-10
+    assert rel_err(lib.array("rho")[:na], o.get("rho")[:na]) < 1e-10
+
+
+def periodic_lattice(npartx=12):
+    x, y, z, dx = workloads.make_lattice(npartx)
+    dy = dx * np.sqrt(3.0) / 2.0
+    dz = dx * np.sqrt(6.0) / 3.0
+    ny = int(round((y.max() - (y.max() % dy)) / dy)) + 1
+    nz = int(round(z.max() / dz)) + 1
+    box = np.array([0.0, 1.0, 0.0, ny * dy, 0.0, nz * dz])
+    rng = np.random.default_rng(5)
+    x = (x + 0.1 * dx * rng.normal(size=x.size)) % 1.0
+    y = (y + 0.1 * dx * rng.normal(size=x.size)) % box[3]
+    z = (z + 0.1 * dx * rng.normal(size=x.size)) % box[5]
+    n = x.size
+    vol = box[1] * box[3] * box[5]
+    c = workloads._pack(x, y, z, np.full(n, 1.2 * (vol / n) ** (1 / 3)), "cubic", "periodic lattice")
+    c["pmass"] = 1.0 / n
+    return c, box
+
+
+def test_periodic_box():
+    c, box = periodic_lattice()
+    n = c["n"]
+    rng = np.random.default_rng(6)
+    v = rng.normal(size=(3, n)) * 0.1
+    lib = Library("cubic")
+    lib.set_option("precision", 0)
+    out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], hfact=1.2, v=v, periodic_box=box)
+    o = O.Oracle(quintic=False)
+    o.set_particles(c["x"], c["y"], c["z"], c["h"], v=v, a=np.zeros((3, n)))
+    o.build(periodic_box=box)
+    assert o.density_solve(c["pmass"], 1e-4, 1.2, 100, 0, True) == 0
+    order = o.get("order")
+    for name in ("h", "rho"):
+        ref = np.empty(n)
+        ref[order] = o.get(name)
+        assert rel_err(out[name], ref) < 1e-10, name
+    ref = np.empty(n)
+    ref[order] = o.get("divv")
+    assert np.max(np.abs(out["divv"] - ref)) < 1e-7 * np.abs(ref).max()
+    np.testing.assert_array_equal(lib.array("jList"), o.get("jList"))
+
+
+def _run(snippet, env=None):
+    code = ("import sys; sys.path.insert(0, %r); sys.path.insert(0, %r)\nimport numpy as np\n"
+            "from cosmosphere_amd import Library, workloads\n" % (ROOT, os.path.join(ROOT, "tests")) + snippet)
+    e = dict(os.environ)
+    e.update(env or {})
+    return subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=180, env=e)
+
+
+def test_force_before_density_aborts():
+    res = _run("lib = Library('cubic'); lib.arrays_init(100); lib.force(100, 0.01)")
+    assert res.returncode == -6
+    assert "FATAL: force_gpu_c called with no GPU density solve for this particle set" in res.stderr
+
+
+def test_download_before_solve_aborts():
+    res = _run("lib = Library('cubic'); lib.arrays_init(50); lib.download(2, 50)")
+    assert res.returncode == -6
+    assert "FATAL: cosmo_download(slot=2) with no ordering" in res.stderr
+
+
+def test_unknown_slot_aborts():
+    res = _run("lib = Library('cubic'); lib.arrays_init(10); lib.L.cosmo_upload(9, None, 10)")
+    assert res.returncode == -6
+    assert "FATAL: cosmo_upload called for slot 9, which has no arrays" in res.stderr
+
+
+def test_unconverged_solve_aborts():
+    # a particle far from the rest sees only itself: h grows by the 20% clamp per iteration
+    # and needs far more than 10 iterations to reach a neighbour
+    res = _run("c = workloads.config2(npartx=6); x = c['x'].copy(); x[0] = 5000.0\n"
+               "lib = Library('cubic'); lib.set_option('max_iter', 10)\n"
+               "lib.solve(x, c['y'], c['z'], c['h'], c['pmass'], hfact=1.2)")
+    assert res.returncode == -6
+    assert "FATAL: solveDensH: unconverged=1 jlist_trunc=0 stack_drops=0 (iters=10)" in res.stderr
+
+
+def test_periodic_support_check_aborts():
+    # 4^3 particles in a unit periodic box: h settles near 0.3, so 2h exceeds half the box
+    res = _run("g = (np.arange(4) + 0.5) / 4\nx, y, z = [a.ravel() for a in np.meshgrid(g, g, g, indexing='ij')]\n"
+               "lib = Library('cubic'); lib.solve(x, y, z, np.full(64, 0.3), 1 / 64, hfact=1.2, "
+               "periodic_box=np.array([0, 1, 0, 1, 0, 1.0]))")
+    assert res.returncode == -6
+    assert "reaches half the periodic box" in res.stderr
+
+
+def test_stats_line():
+    res = _run("c = workloads.config2(npartx=8)\nlib = Library('cubic')\n"
+               "lib.solve(c['x'], c['y'], c['z'], c['h'], c['pmass'], hfact=1.2)", env={"COSMO_DENS_STATS": "1"})
+    assert res.returncode == 0, res.stderr[-400:]
+    line = [l for l in res.stderr.splitlines() if l.startswith("COSMO_STATS")][0]
+    for key in ("n=", "leaves=", "iters=", "bbox=", "keysort=", "tree=", "jbuild=", "nrkern=", "gradkern=", "total="):
+        assert key in line
