@@ -150,6 +150,78 @@ def cpu_baseline(c, gpu_h_fp64, gpu_h_mixed, seconds):
     return base, err
 
 
+def bench_ensemble(args):
+    """C4: 256 independent sets solved as one ensemble (mixed form) against the same sets
+    solved one after another with the plain FP64 single-set path."""
+    t = time.perf_counter()
+    sets = workloads.config4()
+    counts = [s["n"] for s in sets]
+    log("ensemble: %d sets, %d particles (generated in %.1f s)" % (len(sets), sum(counts), time.perf_counter() - t))
+    lib = Library("cubic")
+    # sequential plain FP64 baseline, solve time only (inputs uploaded before each solve)
+    lib.set_option("precision", 0)
+    lib.set_option("closure", 0)
+    seq = 0.0
+    for s in sets:
+        n = s["n"]
+        lib.arrays_init(n)
+        lib.upload(POS, np.stack([s["x"], s["y"], s["z"]]))
+        lib.upload(HSML, s["h"])
+        lib.sync()
+        t0 = time.perf_counter()
+        lib.solve_dens_h(n, s["pmass"], 0, False, None, 1e-4, 1.2)
+        seq += time.perf_counter() - t0
+    log("  sequential fp64: %.1f ms for %d solves" % (1e3 * seq, len(sets)))
+    n = lib.batch_init(counts)
+    lib.upload(POS, np.concatenate([np.stack([s["x"], s["y"], s["z"]]) for s in sets], axis=1))
+    lib.upload(HSML, np.concatenate([s["h"] for s in sets]))
+    lib.upload(VEL, np.zeros((3, n)))
+    lib.upload(ACCEL, np.zeros((3, n)))
+    lib.stash_inputs()
+    pm = [s["pmass"] for s in sets]
+    out = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        lib.set_option("precision", prec)
+        lib.set_option("closure", -1)
+        for _ in range(args.warmup):
+            lib.restore_inputs()
+            lib.batch_solve(pm, 1e-4, 1.2)
+        lib.sync()
+        tot = {"pairs": 0.0, "candidates": 0.0, "nrkern": 0.0, "launches": 0.0, "iters": 0.0}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            lib.restore_inputs()
+            lib.batch_solve(pm, 1e-4, 1.2)
+            for k in tot:
+                tot[k] += lib.stat(k)
+        lib.sync()
+        wall = time.perf_counter() - t0
+        alive = lib.stat("n_alive")
+        flops = FLOPS_CAND * tot["candidates"] + FLOPS_ACC * tot["pairs"]
+        out[name] = {"ms_per_step": 1e3 * wall / args.steps, "h_solves_per_s": alive * args.steps / wall,
+                     "pairs_per_s_kernel": tot["pairs"] / max(tot["nrkern"], 1e-12),
+                     "kernel_tflops": flops / max(tot["nrkern"], 1e-12) / 1e12,
+                     "kernel_avg_ms": 1e3 * tot["nrkern"] / max(tot["launches"], 1),
+                     "iterations": tot["iters"] / args.steps}
+        log("  batched %s: %.1f ms/step, %.3e h-solves/s" % (name, out[name]["ms_per_step"],
+                                                              out[name]["h_solves_per_s"]))
+    m = out["mixed"]
+    res = {
+        "metric": METRIC, "value": m["h_solves_per_s"], "unit": "particle h-solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 pair sums + fp64 Newton", "data": "synthetic",
+        "config": {"workload": "C4 ensemble of %d sets, %d particles" % (len(sets), n), "kernel": "cubic",
+                   "sets": len(sets), "n": n, "parallelism": "single GPU"},
+        "pair_interactions_per_s": m["pairs_per_s_kernel"],
+        "sequential_fp64": {"ms": 1e3 * seq, "h_solves_per_s": sum(counts) / seq},
+        "batched_fp64": out["fp64"],
+        "batched_mixed_vs_sequential_fp64": m["h_solves_per_s"] / (sum(counts) / seq),
+        "roofline": {"bound": "valu", "achieved": m["kernel_tflops"], "peak": FP32_PEAK, "unit": "TFLOP/s",
+                     "frac": m["kernel_tflops"] / FP32_PEAK, "traffic": None, "kernel_avg_ms": m["kernel_avg_ms"]},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +236,9 @@ def main():
         print("bench.py measures one GPU: this solver runs a particle set (or an ensemble of sets) on one MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.config == "C4":
+        bench_ensemble(args)
+        return
 
     t = time.perf_counter()
     c = load_config(args.config)

@@ -30,7 +30,7 @@ ABI_SYMBOLS = ["densityiterate_gpu_c", "cosmo_kernel_radius", "force_gpu_c", "co
                "cosmo_pin_host", "cosmo_unpin_host"]
 EXT_SYMBOLS = ["cosmo_ext_solve_dens_h", "cosmo_ext_set_option", "cosmo_ext_get_stat",
                "cosmo_ext_get_array", "cosmo_ext_stash_inputs", "cosmo_ext_restore_inputs",
-               "cosmo_ext_sync"]
+               "cosmo_ext_sync", "cosmo_batch_init", "cosmo_batch_solve"]
 
 _ARRAY_DTYPES = {"keys": np.uint64, "order": np.int32, "leafStart": np.uint64, "leafLevel": np.int32,
                  "leafSet": np.int32, "layout": np.uint32, "particleLeaf": np.int32,
@@ -68,6 +68,9 @@ def _bind(L):
     L.cosmo_ext_get_stat.restype = ctypes.c_double
     L.cosmo_ext_get_array.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_longlong]
     L.cosmo_ext_get_array.restype = ctypes.c_longlong
+    L.cosmo_batch_init.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.cosmo_batch_solve.argtypes = [c_dp, ctypes.c_int, c_dp, ctypes.c_double, ctypes.c_double]
+    L.cosmo_batch_solve.restype = ctypes.c_int
     return L
 
 
@@ -158,6 +161,13 @@ class Library:
             self.L.cosmo_ext_get_array(name.encode(), out.ctypes.data_as(ctypes.c_void_p), out.nbytes)
         return out
 
+    PASS_FIELDS = ("leaves", "items", "iterating", "candidates", "pairs", "closure_candidates", "kernel_s",
+                   "per_particle")
+
+    def passes(self):
+        """One row per NR pass of the last solve (columns PASS_FIELDS)."""
+        return self.array("passes").reshape(-1, len(self.PASS_FIELDS))
+
     def stash_inputs(self):
         self.L.cosmo_ext_stash_inputs()
 
@@ -166,6 +176,18 @@ class Library:
 
     def sync(self):
         self.L.cosmo_ext_sync()
+
+    # ---- ensemble ----
+    def batch_init(self, counts):
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        self.L.cosmo_batch_init(int(c.size), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return int(c.sum())
+
+    def batch_solve(self, pmass, tolh=1e-4, hfact=-1.0, periodic_boxes=None):
+        pm = np.ascontiguousarray(pmass, dtype=np.float64)
+        bx = np.ascontiguousarray(periodic_boxes if periodic_boxes is not None else np.zeros(6), dtype=np.float64)
+        return self.L.cosmo_batch_solve(_dptr(pm), int(periodic_boxes is not None), _dptr(bx), float(tolh),
+                                        float(hfact))
 
     # ---- convenience: the Phantom call sequence of the reference driver (src/main.cu:118-141) ----
     def solve(self, x, y, z, h, pmass, tolh=1e-4, hfact=-1.0, v=None, a=None, gradients=True, periodic_box=None,

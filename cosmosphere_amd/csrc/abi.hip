@@ -324,6 +324,8 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     if (nm == "precision") s.precision = (int)value;
     else if (nm == "max_iter") s.maxIter = (int)value;
     else if (nm == "mapping") s.mapping = (int)value;
+    else if (nm == "sparse") s.sparse = (int)value;
+    else if (nm == "tail_leaves") s.tailLeaves = (int)value;
     else if (nm == "closure") s.closureOn = (int)value;
     else if (nm == "tree_only") s.treeOnly = (int)value;
     else
@@ -383,7 +385,13 @@ extern "C" long long cosmo_ext_get_array(const char* name, void* dst, long long 
     const std::string nm(name);
     const void* src = nullptr;
     long long count = 0, esz = 8;
-    std::vector<uint64_t> tmp;
+    if (nm == "passes")   // host records: 8 doubles per NR pass of the last solve
+    {
+        const long long cnt = (long long)s.passLog.size() * 8;
+        const long long bytes = std::min(cnt * 8, maxBytes);
+        if (bytes > 0) std::memcpy(dst, s.passLog.data(), (size_t)bytes);
+        return cnt;
+    }
     if (nm == "keys") { src = s.keys.p; count = s.n; }
     else if (nm == "order") { src = s.order.p; count = s.n; esz = 4; }
     else if (nm == "leafStart") { src = s.leafStart.p; count = s.nLeaves; }
@@ -452,6 +460,54 @@ extern "C" void cosmo_ext_restore_inputs(void)
         HIP_CHECK(hipMemcpyAsync(f[k]->as<double>(), st + (size_t)k * n, (size_t)n * 8, hipMemcpyDeviceToDevice,
                                  s.stream));
     HIP_CHECK(hipStreamSynchronize(s.stream));
+}
+
+// =========================================================================================
+// Ensemble: many independent particle sets solved together.  The sets are stored one after
+// another in every slot (set b owns [sum of earlier counts, + counts[b]) of the caller's
+// order); each set has its own root box, tree, particle mass and convergence, and one
+// sequence of kernel launches covers all of them.
+// =========================================================================================
+namespace
+{
+std::vector<int> gBatchCounts;
+}
+
+extern "C" void cosmo_batch_init(int nSets, const int* counts)
+{
+    if (nSets <= 0 || counts == nullptr) fatal("cosmo_batch_init: no sets");
+    long long total = 0;
+    for (int b = 0; b < nSets; ++b)
+    {
+        if (counts[b] < 0) fatal("cosmo_batch_init: negative set size");
+        total += counts[b];
+    }
+    if (total <= 0 || total > 2000000000LL) fatal("cosmo_batch_init: total particle count out of range");
+    gBatchCounts.assign(counts, counts + nSets);
+    sizeParticleArrays(state(), (int)total);
+}
+
+extern "C" int cosmo_batch_solve(const double* pmass, int periodic, const double* boxes, double tolh, double hfact)
+{
+    State& s = state();
+    if (gBatchCounts.empty()) fatal("cosmo_batch_solve: cosmo_batch_init has not been called");
+    long long total = 0;
+    for (int c : gBatchCounts) total += c;
+    requireSized(s, (int)total, "cosmo_batch_solve");
+    ensureStream(s);
+    const int B = (int)gBatchCounts.size();
+    s.n = (int)total;
+    s.nSets = B;
+    s.hSetFirst.assign(B, 0);
+    s.hSetN.assign(gBatchCounts.begin(), gBatchCounts.end());
+    for (int b = 1; b < B; ++b) s.hSetFirst[b] = s.hSetFirst[b - 1] + s.hSetN[b - 1];
+    double* pm = s.pmassSet.get<double>(B + 1);
+    HIP_CHECK(hipMemcpyAsync(pm, pmass, B * sizeof(double), hipMemcpyHostToDevice, s.stream));
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    s.usePmassSet = 1;
+    Timings t = solveDensH(s, pmass[0], 0, false, periodic ? boxes : nullptr, tolh, hfact);
+    s.usePmassSet = 0;
+    return t.iters;
 }
 
 extern "C" void cosmo_ext_sync(void)

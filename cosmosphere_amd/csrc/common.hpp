@@ -37,6 +37,7 @@ constexpr uint64_t kDeadKey   = ~uint64_t(0);
 constexpr int      kWave      = 64;                       // CDNA wavefront
 constexpr int      kBucket    = 64;                       // leaf bucket size (tree parity size)
 constexpr double   kPi        = 3.141592653589793;
+constexpr double   kFinestCell = 1.0 / 2097152.0;         // level-21 cell per unit box length
 
 #ifdef COSMO_KERNEL_QUINTIC
 constexpr double kRad    = 3.0;
@@ -84,6 +85,43 @@ __host__ __device__ inline void kernWdW(T q2, T q, T& w, T& dw)
     const T a2 = a * a, a3 = a2 * a, b2 = b * b, b3 = b2 * b;
     w  = T(0.25) * (a3 - T(4) * b3);
     dw = T(0.25) * (T(-3) * a2 + T(12) * b2);
+#endif
+}
+
+// Branch-free pair terms of the mixed form (a wavefront whose q values straddle several
+// pieces would otherwise execute every piece), FP32, for 0 <= q <= R.  Both kernels are sums of
+// truncated powers (the reference's piecewise forms expanded, pinned by the CPU tests):
+//   quintic  W = a^5 - 6 b^5 + 15 c^5,  dW/dq = -5 (a^4 - 6 b^4 + 15 c^4),
+//            a = 3-q, b = (2-q)_+, c = (1-q)_+
+//   cubic    W = (a^3 - 4 b^3) / 4,     dW/dq = -3/4 (a^2 - 4 b^2),  a = 2-q, b = (1-q)_+
+// With V the bracket of W and D the bracket of dW/dq:
+//   W = kWw V   and   -q dW/dq - 3 W = kGp q D - kGw V,
+// so a pair loop accumulates V and q D and applies the constants once per particle.  The
+// coefficients are exact in FP32, and so is the evaluation at q = 0 (the self pair of every
+// particle): V(0) = 66 (quintic) or 4 (cubic), D(0) = 0.  a = R - q needs no clamp: q <= R.
+#ifdef COSMO_KERNEL_QUINTIC
+constexpr double kWw = 1.0, kGp = 5.0, kGw = 3.0;
+#else
+constexpr double kWw = 0.25, kGp = 0.75, kGw = 0.75;
+#endif
+
+template<class T>
+__host__ __device__ inline void kernVD(T q, T& V, T& D)
+{
+#ifdef COSMO_KERNEL_QUINTIC
+    const T a = T(3) - q;
+    const T b = fmax(T(2) - q, T(0));
+    const T c = fmax(T(1) - q, T(0));
+    const T a2 = a * a, b2 = b * b, c2 = c * c;
+    const T a4 = a2 * a2, b4 = b2 * b2, c4 = c2 * c2;
+    V = fma(T(15), c4 * c, fma(T(-6), b4 * b, a4 * a));
+    D = fma(T(15), c4, fma(T(-6), b4, a4));
+#else
+    const T a = T(2) - q;
+    const T b = fmax(T(1) - q, T(0));
+    const T a2 = a * a, b2 = b * b;
+    V = fma(T(-4), b2 * b, a2 * a);
+    D = fma(T(-4), b2, a2);
 #endif
 }
 

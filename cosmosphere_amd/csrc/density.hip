@@ -1,6 +1,6 @@
 /*
- * density.hip -- the smoothing-length Newton-Raphson density kernel, its closure, the
- * active-set compaction and the post-convergence gradient sweep.
+ * density.hip -- the post-convergence gradient sweep and small helpers of the solve (the
+ * Newton-Raphson pass itself is in nr.hip).
  *
  * Restates src/density_base.cu:237-477 (density sums and NR update), :479-714 (gradient
  * sweep) and :856-886 (compaction) for gfx950.
@@ -37,240 +37,6 @@ namespace csph
 
 constexpr int kDensWaves   = 4;
 constexpr int kDensThreads = kDensWaves * kWave;
-
-// The reference's NR update, in FP64: src/density_base.cu:294-333.
-__device__ inline void nrUpdate(double wsum, double gsum, double pmass, double tolh, double hfact, double& h,
-                                double& rhoOut, double& gradOut, bool& conv)
-{
-#pragma clang fp contract(off)
-    const double hi = h, hiOld = h;
-    const double hi1 = 1.0 / hi, hi31 = hi1 * hi1 * hi1, hi41 = hi31 * hi1;
-    const double rho_i  = wsum * kCnormk * pmass * hi31;
-    const double grad_i = gsum * kCnormk * pmass * hi41;
-    if (!(rho_i > 0.0))
-    {
-        rhoOut = 0.0; gradOut = 0.0; conv = false;
-        return;
-    }
-    const double rhoh_i  = pmass * (hfact * hi1) * (hfact * hi1) * (hfact * hi1);
-    const double funci   = rhoh_i - rho_i;
-    const double dhdrhoi = -hi / (3.0 * rhoh_i);
-    const double omegai  = 1.0 - dhdrhoi * grad_i;
-    const double safe    = (omegai > 0.0) ? omegai : fabs(omegai + 1e-300);
-    const double hraw    = hi - funci * dhdrhoi / safe;
-    double hnew;
-    if      (hraw > 1.2 * hi) hnew = 1.2 * hi;
-    else if (hraw < 0.8 * hi) hnew = 0.8 * hi;
-    else                      hnew = hraw;
-    rhoOut  = rho_i;
-    gradOut = grad_i;
-    h       = hnew;
-    conv    = fabs((hnew - hi) / hiOld) < tolh;
-}
-
-__device__ inline void blockCounters(unsigned long long* __restrict__ counters, unsigned long long v0,
-                                     unsigned long long v1)
-{
-    __shared__ unsigned long long red[2][kDensWaves];
-    v0 = waveReduceSum(v0);
-    v1 = waveReduceSum(v1);
-    const int w = threadIdx.x >> 6;
-    if (laneId() == 0) { red[0][w] = v0; red[1][w] = v1; }
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        unsigned long long s0 = 0, s1 = 0;
-        for (int k = 0; k < kDensWaves; ++k) { s0 += red[0][k]; s1 += red[1][k]; }
-        unsigned long long* slot = counters + (blockIdx.x % kCounterSlots) * kCounterStride;
-        if (s0) atomicAdd(slot + 0, s0);
-        if (s1) atomicAdd(slot + 1, s1);
-    }
-}
-
-template<class Real, bool PER, int MODE>
-__global__ __launch_bounds__(kDensThreads) void densityKernel(DensArgs A)
-{
-    constexpr bool kMixed = sizeof(Real) == 4;
-    // mixed: j offsets as 32-bit fixed point from the i-leaf centre; plain: FP64 positions
-    using TileT = typename std::conditional<kMixed, int, double>::type;
-    __shared__ int   segStart[kDensWaves][kWave];
-    __shared__ int   segJ[kDensWaves][kWave];
-    __shared__ TileT tx[kDensWaves][kWave], ty[kDensWaves][kWave], tz[kDensWaves][kWave];
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = laneId();
-    const int a = blockIdx.x * kDensWaves + w;
-    unsigned long long nAcc = 0, nCand = 0;
-
-    if (a < A.nLeaves)
-    {
-        const int iLeaf = A.leaves ? A.leaves[a] : a;
-        const uint32_t p0 = A.layout[iLeaf], p1 = A.layout[iLeaf + 1];
-        const uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
-        DBox box{};
-        if constexpr (PER) box = A.boxes[A.leafSet[iLeaf]];
-        // Mixed form: offsets from the i-leaf centre c in units of range / 2^29, where range
-        // = leaf half-size + R hmax bounds every i offset.  The difference of two such
-        // integers is exact, so a pair separation is rounded once, to FP32, relative to
-        // itself (not to the offsets).  j offsets are clamped to +-2^30: a clamped j is at
-        // least `range` from every i, beyond R h_i, and differences stay inside int32.
-        double c0 = 0.0, c1 = 0.0, c2 = 0.0, S = 0.0, invS2 = 0.0;
-        if constexpr (kMixed)
-        {
-            const int node = A.leafNode[iLeaf];
-            c0 = A.cx[node]; c1 = A.cy[node]; c2 = A.cz[node];
-            double hm = A.hmaxLeaf[iLeaf];
-            if (!isfinite(hm) || hm <= 0.0) hm = 0.0;
-            const double range = fmax(fmax(A.sx[node], A.sy[node]), A.sz[node]) + kRad * hm;
-            S = 536870912.0 / range;   // 2^29
-            invS2 = 1.0 / (S * S);
-        }
-        auto toFixed = [&](double v) -> int {
-            v = rint(v * S);
-            v = fmin(fmax(v, -1073741824.0), 1073741824.0);   // +-2^30
-            return (int)v;
-        };
-        for (uint32_t q0 = p0; q0 < p1; q0 += kWave)
-        {
-            const uint32_t i     = q0 + lane;
-            const bool     valid = i < p1;
-            const bool     doLane = valid && (MODE != 0 || A.active[i]);
-            const uint64_t doMask = ballot64(doLane);
-            if (doMask == 0) continue;
-            const double xi = valid ? A.x[i] : 0.0, yi = valid ? A.y[i] : 0.0, zi = valid ? A.z[i] : 0.0;
-            double hcur = valid ? A.h[i] : 1.0;
-            const double hsi = 1.0 / (hcur * hcur);
-            TileT xr, yr, zr;
-            if constexpr (kMixed) { xr = toFixed(xi - c0); yr = toFixed(yi - c1); zr = toFixed(zi - c2); }
-            else                  { xr = xi; yr = yi; zr = zi; }
-            // q^2 = r^2 h^-2 in fixed-point units: hi part for the candidate test, the
-            // remainder refines it for accepted pairs
-            const double hsd = kMixed ? hsi * invS2 : hsi;
-            const Real hsr = Real(hsd);
-            const Real hsrLo = kMixed ? Real(hsd - (double)hsr) : Real(0);
-            // Sums are FP64 in both forms: the mixed form evaluates each pair term in FP32
-            // and adds it to an FP64 accumulator (full rate on this GPU, cheaper than FP32
-            // compensated summation and exact to the rounding of the terms themselves).
-            double sw = 0, sg = 0;
-            int  acc = 0;
-            uint32_t cand = 0;
-            auto load = [&](int l, int j) {
-                double xj = A.x[j], yj = A.y[j], zj = A.z[j];
-                if constexpr (kMixed)
-                {
-                    xj -= c0; yj -= c1; zj -= c2;
-                    if constexpr (PER)
-                    {
-                        if (fabs(xj) > 0.5 * box.l[0]) xj -= box.l[0] * copysign(1.0, xj);
-                        if (fabs(yj) > 0.5 * box.l[1]) yj -= box.l[1] * copysign(1.0, yj);
-                        if (fabs(zj) > 0.5 * box.l[2]) zj -= box.l[2] * copysign(1.0, zj);
-                    }
-                    tx[w][l] = toFixed(xj); ty[w][l] = toFixed(yj); tz[w][l] = toFixed(zj);
-                }
-                else
-                {
-                    tx[w][l] = xj; ty[w][l] = yj; tz[w][l] = zj;
-                }
-            };
-            auto consume = [&](int nt) {
-#pragma unroll 4
-                for (int k = 0; k < nt; ++k)
-                {
-                    Real dx, dy, dz;
-                    if constexpr (kMixed)
-                    {
-                        dx = (Real)(xr - tx[w][k]); dy = (Real)(yr - ty[w][k]); dz = (Real)(zr - tz[w][k]);
-                    }
-                    else
-                    {
-                        dx = xr - tx[w][k]; dy = yr - ty[w][k]; dz = zr - tz[w][k];
-                        if constexpr (PER)
-                        {
-                            if (fabs(dx) > 0.5 * box.l[0]) dx = dx - box.l[0] * copysign(1.0, dx);
-                            if (fabs(dy) > 0.5 * box.l[1]) dy = dy - box.l[1] * copysign(1.0, dy);
-                            if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
-                        }
-                    }
-                    const Real r2 = dx * dx + dy * dy + dz * dz;
-                    Real q2 = r2 * hsr;
-                    if (q2 < Real(kRad2))
-                    {
-                        Real q;
-                        if constexpr (kMixed)
-                        {
-                            q2 = fmaf(r2, hsrLo, q2);
-                            q = __builtin_amdgcn_sqrtf(q2);
-                        }
-                        else
-                        {
-                            q = sqrt(q2);
-                        }
-                        Real wk, dwk;
-                        kernWdW<Real>(q2, q, wk, dwk);
-                        sw += (double)wk;                               // list order, as the reference
-                        sg += (double)(-q * dwk - Real(3) * wk);
-                        ++acc;
-                    }
-                }
-                cand += (uint32_t)nt;
-            };
-            forEachTile(segStart[w], segJ[w], A.layout, A.jList, lo, hi, load, consume);
-
-            const double wsum = sw;
-            const double gsum = sg;
-            if (doLane)
-            {
-                nAcc += (unsigned long long)acc;
-                nCand += cand;
-                if constexpr (MODE == 2)
-                {
-#pragma clang fp contract(off)
-                    const double hi1 = 1.0 / hcur, hi31 = hi1 * hi1 * hi1, hi41 = hi31 * hi1;
-                    A.rho[i]   = wsum * kCnormk * A.pmass * hi31;
-                    A.gradh[i] = gsum * kCnormk * A.pmass * hi41;
-                }
-                else
-                {
-                    double r, g;
-                    bool   conv;
-                    nrUpdate(wsum, gsum, A.pmass, A.tolh, A.hfact, hcur, r, g, conv);
-                    A.rho[i]   = r;
-                    A.gradh[i] = g;
-                    A.h[i]     = hcur;
-                    if (MODE == 0) A.active[i] = conv ? 0 : 1;
-                    else if (conv) A.active[i] = 0;
-                }
-            }
-        }
-    }
-    blockCounters(A.counters, nAcc, nCand);
-}
-
-// Active-leaf compaction: flag[a] = leaf a still holds an active particle; counts the
-// active particles.
-__global__ __launch_bounds__(256) void leafFlagKernel(const int* __restrict__ leaves, int nLeaves,
-                                                      const uint32_t* __restrict__ layout,
-                                                      const uint8_t* __restrict__ active, uint32_t* __restrict__ flag,
-                                                      unsigned long long* __restrict__ nActive)
-{
-    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (a >= nLeaves) return;
-    const int l = leaves[a];
-    const uint32_t p0 = layout[l], p1 = layout[l + 1];
-    unsigned long long c = 0;
-    for (uint32_t p = p0 + laneId(); p < p1; p += kWave) c += active[p] ? 1 : 0;
-    c = waveReduceSum(c);
-    if (laneId() == 0)
-    {
-        flag[a] = c ? 1u : 0u;
-        if (c) atomicAdd(nActive + (blockIdx.x % kCounterSlots) * kCounterStride, c);
-    }
-}
-
-__global__ void leafScatterKernel(const int* __restrict__ leaves, int nLeaves, const uint32_t* __restrict__ flag,
-                                  const uint32_t* __restrict__ pos, int* __restrict__ out)
-{
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < nLeaves && flag[a]) out[pos[a]] = leaves[a];
-}
 
 __global__ void initActiveKernel(uint8_t* __restrict__ active, int nAlive, int n, double* __restrict__ rho,
                                  double* __restrict__ gradh)
@@ -444,25 +210,6 @@ __global__ void deadGradientKernel(int first, int n, double* __restrict__ divv, 
 // ---------------------------------------------------------------------------------------
 // Host wrappers
 // ---------------------------------------------------------------------------------------
-void launchDensity(const DensArgs& a, int precision, int mode, bool periodic, hipStream_t st)
-{
-    if (a.nLeaves <= 0) return;
-    const int grid = iceil(a.nLeaves, kDensWaves);
-#define CSPH_DENS(R, P, M) densityKernel<R, P, M><<<grid, kDensThreads, 0, st>>>(a)
-    if (precision == kMixed)
-    {
-        if (periodic) { if (mode == 0) CSPH_DENS(float, true, 0); else if (mode == 1) CSPH_DENS(float, true, 1); else CSPH_DENS(float, true, 2); }
-        else          { if (mode == 0) CSPH_DENS(float, false, 0); else if (mode == 1) CSPH_DENS(float, false, 1); else CSPH_DENS(float, false, 2); }
-    }
-    else
-    {
-        if (periodic) { if (mode == 0) CSPH_DENS(double, true, 0); else if (mode == 1) CSPH_DENS(double, true, 1); else CSPH_DENS(double, true, 2); }
-        else          { if (mode == 0) CSPH_DENS(double, false, 0); else if (mode == 1) CSPH_DENS(double, false, 1); else CSPH_DENS(double, false, 2); }
-    }
-#undef CSPH_DENS
-    LAUNCH_CHECK();
-}
-
 void launchGradients(const GradArgs& a, bool periodic, hipStream_t st)
 {
     if (a.nLeaves > 0)
@@ -481,25 +228,6 @@ void launchDeadGradients(int first, int n, double* divv, double* xi, double* ddi
         deadGradientKernel<<<iceil(n - first, 256), 256, 0, st>>>(first, n, divv, xi, ddivvdt);
         LAUNCH_CHECK();
     }
-}
-
-int counterSlots() { return kCounterSlots; }
-int counterStride() { return kCounterStride; }
-
-void launchLeafFlags(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* active, uint32_t* flag,
-                     unsigned long long* nActive, hipStream_t st)
-{
-    if (nLeaves <= 0) return;
-    leafFlagKernel<<<iceil(nLeaves, 4), 256, 0, st>>>(leaves, nLeaves, layout, active, flag, nActive);
-    LAUNCH_CHECK();
-}
-
-void launchLeafScatter(const int* leaves, int nLeaves, const uint32_t* flag, const uint32_t* pos, int* out,
-                       hipStream_t st)
-{
-    if (nLeaves <= 0) return;
-    leafScatterKernel<<<iceil(nLeaves, 256), 256, 0, st>>>(leaves, nLeaves, flag, pos, out);
-    LAUNCH_CHECK();
 }
 
 void launchInitActive(uint8_t* active, int nAlive, int n, double* rho, double* gradh, hipStream_t st)

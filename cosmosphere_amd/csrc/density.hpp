@@ -1,6 +1,7 @@
 /*
- * density.hpp -- launch interface of the density, closure, compaction and gradient kernels
- * (density.hip) and of the force kernels (force.hip).
+ * density.hpp -- launch interface of the Newton-Raphson pass (nr.hip: density kernel,
+ * closure, work items, compaction), of the gradient sweep (density.hip) and of the force
+ * kernels (force.hip).
  */
 #pragma once
 
@@ -13,6 +14,9 @@ struct DensArgs
 {
     const int*      leaves;     // active leaf ids (nullptr: leaf a is leaf a)
     int             nLeaves;
+    const int*      itemPos;    // work item -> position in `leaves` (nullptr: item k is position k)
+    const int*      itemPart;   // work item -> particle, or -1 for a whole-leaf item (nullptr: all -1)
+    int             nItems;
     const uint32_t* jOffset;    // CSR of the lists, indexed by position in `leaves`
     const int*      jList;
     const uint32_t* layout;
@@ -25,10 +29,25 @@ struct DensArgs
     double*         h;
     double*         rho;
     double*         gradh;
-    uint8_t*        active;
+    uint8_t*        state;      // 1 iterating, 2 converged with closure pending, 0 done
     const DBox*     boxes;
     double          pmass, tolh, hfact;
+    const double*   pmassSet;   // ensemble: particle mass of each set (nullptr: pmass)
+    int             closure;    // converged particles pass through state 2
     unsigned long long* counters;
+};
+
+// Device work counters (u64, summed over the counter slots).
+enum Counter
+{
+    kCtrPairs = 0,        // accepted pairs of NR evaluations
+    kCtrCand = 1,         // candidate pairs of NR evaluations
+    kCtrIterating = 2,    // particles in state 1 after the pass
+    kCtrRemaining = 3,    // particles in state != 0 after the pass
+    kCtrItems = 4,        // work items of the next pass
+    kCtrClosPairs = 5,    // accepted pairs of closure evaluations
+    kCtrClosCand = 6,     // candidate pairs of closure evaluations
+    kNumCounters = 7
 };
 
 struct GradArgs
@@ -67,15 +86,22 @@ struct ForceArgs
     unsigned long long* counters;
 };
 
-// mode 0: NR on active particles (FLAT_PARTICLE); 1: NR on every particle of an active
-// leaf (WARP_PER_LEAF); 2: closure, rho and d(rho)/d(h) at the current h, no update.
+// mode 0: NR on the particles in state != 0 (FLAT_PARTICLE; state 2 = closure only);
+// 1: NR on every particle of an active leaf (WARP_PER_LEAF); 2: closure of every particle,
+// rho and d(rho)/d(h) at the current h, no update.  a.nItems work items.
 void launchDensity(const DensArgs& a, int precision, int mode, bool periodic, hipStream_t st);
 void launchGradients(const GradArgs& a, bool periodic, hipStream_t st);
 void launchDeadGradients(int first, int n, double* divv, double* xi, double* ddivvdt, hipStream_t st);
-void launchLeafFlags(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* active, uint32_t* flag,
-                     unsigned long long* nActive, hipStream_t st);
-void launchLeafScatter(const int* leaves, int nLeaves, const uint32_t* flag, const uint32_t* pos, int* out,
-                       hipStream_t st);
+// Per active leaf: flag (has a particle in state != 0) and its work-item count (1 whole-leaf
+// item, or one item per remaining particle when at most `sparse` remain); counters 2-4.
+void launchLeafFlags(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* state, int sparse,
+                     uint32_t* flag, uint32_t* itemCount, unsigned long long* counters, hipStream_t st);
+// Compaction: out[pos[a]] = leaves[a], outCount[pos[a]] = count[a] for flagged a.
+void launchLeafScatter(const int* leaves, const uint32_t* count, int nLeaves, const uint32_t* flag,
+                       const uint32_t* pos, int* out, uint32_t* outCount, hipStream_t st);
+// Work items of the active leaves from their counts' exclusive scan `off`.
+void launchItemFill(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* state, int sparse,
+                    const uint32_t* off, int* itemPos, int* itemPart, hipStream_t st);
 void launchInitActive(uint8_t* active, int nAlive, int n, double* rho, double* gradh, hipStream_t st);
 void launchIota(int* v, int n, hipStream_t st);
 void launchMaxH(const double* h, int n, unsigned long long* out, hipStream_t st);

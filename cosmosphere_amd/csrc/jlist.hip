@@ -15,7 +15,18 @@
  *
  * The accept tests are evaluated exactly as written (no contraction), which keeps the
  * lists bit-identical to the FP64 oracle's.
+ *
+ * Tight lists (Newton-Raphson passes).  An i-leaf whose particles still to be evaluated all
+ * have h below its hmax needs only the j-leaves within R max(h_remaining) of its box.  A
+ * second test drops the nodes beyond that reach (Euclidean box gap, both boxes widened by
+ * one finest cell: a particle lies within one finest cell of its node's box, since the key
+ * quantisation floor(x m) - lo m truncates up to one cell low; relative slack 1e-5).  A node
+ * it drops holds no pair inside any remaining particle's support, and a dropped internal
+ * node's leaves are dropped with it, so the tight list is the reference list minus leaves
+ * without such pairs, in the same depth-first order: the accepted pairs and their order are
+ * unchanged.
  */
+#include "tile.hpp"
 #include "tree.hpp"
 
 #include <vector>
@@ -23,24 +34,35 @@
 namespace csph
 {
 
-// hmax_leaf = max finite h of the leaf's particles: include/tree.cuh:177-191.
+// hmax_leaf = max finite h of the leaf's particles: include/tree.cuh:177-191.  With `state`,
+// also hRem = max finite h of the particles in state != 0 (the tight-list reach).
 // One wavefront per leaf, 4 per block.
 __global__ __launch_bounds__(256) void hmaxLeafKernel(const double* __restrict__ h, const uint32_t* __restrict__ layout,
                                                       const int* __restrict__ leaves, int nLeaves,
-                                                      double* __restrict__ hmaxLeaf)
+                                                      double* __restrict__ hmaxLeaf, const uint8_t* __restrict__ state,
+                                                      double* __restrict__ hRem)
 {
     const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (a >= nLeaves) return;
     const int l = leaves ? leaves[a] : a;
     const uint32_t p0 = layout[l], p1 = layout[l + 1];
-    double m = 0.0;
+    double m = 0.0, r = 0.0;
     for (uint32_t p = p0 + laneId(); p < p1; p += kWave)
     {
         const double v = h[p];
-        if (isfinite(v)) m = fmax(m, v);
+        if (isfinite(v))
+        {
+            m = fmax(m, v);
+            if (state && state[p] != 0) r = fmax(r, v);
+        }
     }
     m = waveReduceMax(m);
     if (laneId() == 0) hmaxLeaf[l] = m;
+    if (state)
+    {
+        r = waveReduceMax(r);
+        if (laneId() == 0) hRem[l] = r;
+    }
 }
 
 // Upsweep: include/tree.cuh:196-217 (a leaf node takes its hmax_leaf if finite and > 0,
@@ -104,6 +126,7 @@ struct ListArgs
     const double*   cx; const double* cy; const double* cz;
     const double*   sx; const double* sy; const double* sz;
     const double*   hmaxLeaf;
+    const double*   hRemLeaf;   // tight lists: reach of the particles still to evaluate (nullptr: off)
     const double*   nodeHmax;
     const DBox*     boxes;
     const uint32_t* jOffset;
@@ -135,6 +158,15 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
     const double radHi = kRad * hmax;
     const double aSize[3] = {SYM ? is[0] : is[0] + radHi, SYM ? is[1] : is[1] + radHi, SYM ? is[2] : is[2] + radHi};
 
+    // tight-list reach, squared (negative: no second test)
+    double tight2 = -1.0;
+    if (!SYM && A.hRemLeaf)
+    {
+        const double hr = A.hRemLeaf[iLeaf];
+        if (hr < hmax) tight2 = (kRad * hr) * (kRad * hr) * (1.0 + 1e-5);
+    }
+    const double pad[3] = {2.0 * kFinestCell * box.l[0], 2.0 * kFinestCell * box.l[1], 2.0 * kFinestCell * box.l[2]};
+
     auto inRange = [&](int node) -> bool {
         const double d2 = boxDist2<PER>(ic, aSize, A.cx[node], A.cy[node], A.cz[node], A.sx[node], A.sy[node],
                                         A.sz[node], box);
@@ -145,7 +177,12 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
         }
         else
         {
-            return d2 <= 0.0;
+            if (!(d2 <= 0.0)) return false;
+            if (tight2 < 0.0) return true;
+            const double isPad[3] = {is[0] + pad[0], is[1] + pad[1], is[2] + pad[2]};
+            const double g2 = boxDist2<PER>(ic, isPad, A.cx[node], A.cy[node], A.cz[node], A.sx[node], A.sy[node],
+                                            A.sz[node], box);
+            return g2 <= tight2;
         }
     };
 
@@ -212,12 +249,13 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
 // ---------------------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------------------
-void hmaxLeaves(State& s, const int* leaves, int nLeaves)
+void hmaxLeaves(State& s, const int* leaves, int nLeaves, const uint8_t* state)
 {
     if (nLeaves <= 0) return;
     double* hm = s.hmaxLeaf.get<double>(s.nLeaves + 1);
+    double* hr = state ? s.hRemLeaf.get<double>(s.nLeaves + 1) : nullptr;
     hmaxLeafKernel<<<iceil(nLeaves, 4), 256, 0, s.stream>>>(s.h.as<double>(), s.layout.as<uint32_t>(), leaves, nLeaves,
-                                                             hm);
+                                                             hm, state, hr);
     LAUNCH_CHECK();
 }
 
@@ -245,7 +283,7 @@ static void launchList(const ListArgs& a, bool periodic, hipStream_t st)
     LAUNCH_CHECK();
 }
 
-long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric)
+long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric, bool tight)
 {
     hipStream_t st = s.stream;
     uint32_t* jc = s.jCount.get<uint32_t>(nLeaves + 1);
@@ -260,6 +298,7 @@ long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric)
     a.cx = s.nodeCx.as<double>(); a.cy = s.nodeCy.as<double>(); a.cz = s.nodeCz.as<double>();
     a.sx = s.nodeSx.as<double>(); a.sy = s.nodeSy.as<double>(); a.sz = s.nodeSz.as<double>();
     a.hmaxLeaf = s.hmaxLeaf.as<double>();
+    a.hRemLeaf = (tight && !symmetric) ? s.hRemLeaf.as<double>() : nullptr;
     a.nodeHmax = symmetric ? s.nodeHmax.as<double>() : nullptr;
     a.boxes = s.box.as<DBox>();
     a.jCount = jc;
@@ -271,7 +310,8 @@ long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric)
         if (symmetric) launchList<true, true>(a, per, st);
         else           launchList<false, true>(a, per, st);
     }
-    unsigned long long* tot = s.counters.get<unsigned long long>(64) + 32;
+    unsigned long long* tot =
+        s.counters.get<unsigned long long>((size_t)kCounterSlots * kCounterStride + 64) + (size_t)kCounterSlots * kCounterStride + 16;
     fillU32(reinterpret_cast<uint32_t*>(tot), 0u, 2, st);
     scanExclusive<uint32_t>(jc, jo, nLeaves, s.scanTmp, tot, st);
     unsigned long long total = 0;

@@ -1,0 +1,578 @@
+/*
+ * nr.hip -- one Newton-Raphson pass of the smoothing-length solve: the density kernel with its
+ * fused NR update and closure, the work items, and the compaction of the active set.
+ *
+ * Restates src/density_base.cu:237-477 (density sums and NR update) and :856-886 (compaction)
+ * for gfx950.
+ *
+ * Work items.  Each active leaf becomes either
+ *   - one whole-leaf item: one wavefront; the j-particles of the leaf's list are staged 64 at a
+ *     time in an LDS tile (coalesced loads, once per wavefront) and the c particles still to
+ *     evaluate (up to 64 at a time) each take g = floor(64 / c) lanes, lane s of a group
+ *     sweeping tile slots s, s+g, ... with broadcast reads; a group reduction closes each
+ *     particle.  A leaf of more than 32 runs lane = particle (g = 1, list order); a leaf with
+ *     20 particles left uses 60 lanes and sweeps each tile in 22 steps instead of 64; or
+ *   - in the last passes (few active leaves, so the chip would sit mostly idle), one item per
+ *     particle of a leaf with at most `sparse` left: one wavefront per particle, the 64 lanes
+ *     splitting its j-candidates, then a wavefront reduction.  The unconverged particles of
+ *     the last passes are isolated ones whose h reaches into dense regions, with lists of up
+ *     to millions of candidates: spreading them over wavefronts shortens the pass.
+ *
+ * Pruning.  The lists of a FLAT pass are the reference's lists trimmed to the reach of the
+ * particles still to evaluate (jlist.hip, tight lists).  A per-particle item further skips
+ * the list entries whose leaf box lies beyond R h_i of its particle.  Skipped leaves hold no
+ * pair with q < R (the test keeps two finest cells and a 1e-5 relative margin: a particle
+ * lies within one finest cell of its node's box, since the key quantisation floor(x m) - lo m
+ * truncates up to one cell low), so the accepted pairs, and in the plain form the sums of a
+ * whole-leaf item, are unchanged.
+ *
+ * Per-particle state (u8): 1 = still iterating, 2 = converged, closure pending (rho and
+ * d(rho)/d(h) are evaluated once more, at the final h, in the next pass), 0 = done.  With the
+ * closure off (the reference's density-only behaviour) a converged particle goes straight to 0
+ * and keeps the rho of its last update, taken at the h before it (src/density_base.cu:299-333).
+ *
+ * Precision forms:
+ *   plain  FP64 throughout, the reference's arithmetic; whole-leaf items sum pair terms in list
+ *          order, per-particle items by a wavefront tree;
+ *   mixed  pair separations are formed exactly and rounded once to FP32 -- 32-bit fixed-point
+ *          offsets from the i-leaf centre in whole-leaf items, an FP64 difference in
+ *          per-particle items -- then r^2, q, the kernel and the square root in FP32; each pair
+ *          term is added to an FP64 accumulator; the NR update, the convergence test and the
+ *          closure's rho(h) formula are FP64.
+ */
+#include "density.hpp"
+#include "prims.hpp"
+#include "tile.hpp"
+
+#include <type_traits>
+
+namespace csph
+{
+
+constexpr int    kNrWaves    = 4;
+constexpr int    kNrThreads  = kNrWaves * kWave;
+constexpr double kPruneSlack = 1.0 + 1e-5;
+constexpr int    kFarFixed   = 1 << 30;           // padding j of the fixed-point tiles
+
+// The reference's NR update, in FP64: src/density_base.cu:294-333.
+__device__ inline void nrUpdate(double wsum, double gsum, double pmass, double tolh, double hfact, double& h,
+                                double& rhoOut, double& gradOut, bool& conv)
+{
+#pragma clang fp contract(off)
+    const double hi = h, hiOld = h;
+    const double hi1 = 1.0 / hi, hi31 = hi1 * hi1 * hi1, hi41 = hi31 * hi1;
+    const double rho_i  = wsum * kCnormk * pmass * hi31;
+    const double grad_i = gsum * kCnormk * pmass * hi41;
+    if (!(rho_i > 0.0))
+    {
+        rhoOut = 0.0; gradOut = 0.0; conv = false;
+        return;
+    }
+    const double rhoh_i  = pmass * (hfact * hi1) * (hfact * hi1) * (hfact * hi1);
+    const double funci   = rhoh_i - rho_i;
+    const double dhdrhoi = -hi / (3.0 * rhoh_i);
+    const double omegai  = 1.0 - dhdrhoi * grad_i;
+    const double safe    = (omegai > 0.0) ? omegai : fabs(omegai + 1e-300);
+    const double hraw    = hi - funci * dhdrhoi / safe;
+    double hnew;
+    if      (hraw > 1.2 * hi) hnew = 1.2 * hi;
+    else if (hraw < 0.8 * hi) hnew = 0.8 * hi;
+    else                      hnew = hraw;
+    rhoOut  = rho_i;
+    gradOut = grad_i;
+    h       = hnew;
+    conv    = fabs((hnew - hi) / hiOld) < tolh;
+}
+
+template<int MODE>
+__device__ inline bool isClosureEval(uint8_t st) { return MODE == 2 || (MODE == 0 && st == 2); }
+
+// The end of one particle's evaluation: NR update or closure, and its next state.
+template<int MODE>
+__device__ inline void finishParticle(const DensArgs& A, uint32_t i, int iLeaf, uint8_t st, double wsum, double gsum,
+                                      double hcur)
+{
+    const double pmass = A.pmassSet ? A.pmassSet[A.leafSet[iLeaf]] : A.pmass;
+    if (isClosureEval<MODE>(st))
+    {
+#pragma clang fp contract(off)
+        const double hi1 = 1.0 / hcur, hi31 = hi1 * hi1 * hi1, hi41 = hi31 * hi1;
+        A.rho[i]   = wsum * kCnormk * pmass * hi31;
+        A.gradh[i] = gsum * kCnormk * pmass * hi41;
+        if (MODE == 0) A.state[i] = 0;
+        return;
+    }
+    double r, g;
+    bool   conv;
+    nrUpdate(wsum, gsum, pmass, A.tolh, A.hfact, hcur, r, g, conv);
+    A.rho[i]   = r;
+    A.gradh[i] = g;
+    A.h[i]     = hcur;
+    if (MODE == 0)            A.state[i] = conv ? (A.closure ? 2 : 0) : 1;
+    else if (st != 0 && conv) A.state[i] = 0;   // WARP_PER_LEAF: converged particles are recomputed
+}
+
+// Block sum of four counters, one atomic each per block.
+__device__ inline void blockCounters(unsigned long long* __restrict__ counters, const unsigned long long v[4])
+{
+    __shared__ unsigned long long red[4][kNrWaves];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const unsigned long long s = waveReduceSum(v[k]);
+        if (laneId() == 0) red[k][w] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4)
+    {
+        const int idx[4] = {kCtrPairs, kCtrCand, kCtrClosPairs, kCtrClosCand};
+        unsigned long long s = 0;
+        for (int q = 0; q < kNrWaves; ++q) s += red[threadIdx.x][q];
+        if (s) atomicAdd(counters + (blockIdx.x % kCounterSlots) * kCounterStride + idx[threadIdx.x], s);
+    }
+}
+
+template<bool PER>
+__device__ inline void imageFP64(double& dx, double& dy, double& dz, const DBox& box)
+{
+    if constexpr (PER)
+    {
+        if (fabs(dx) > 0.5 * box.l[0]) dx = dx - box.l[0] * copysign(1.0, dx);
+        if (fabs(dy) > 0.5 * box.l[1]) dy = dy - box.l[1] * copysign(1.0, dy);
+        if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
+    }
+}
+
+// Squared distance from the box (c, half-size s) to leaf jl's node box, both widened by one
+// finest cell, nearest periodic image per axis.
+template<bool PER>
+__device__ inline double gapToLeaf(const DensArgs& A, const DBox& box, double c0, double c1, double c2, double s0,
+                                   double s1, double s2, int jl)
+{
+    const int node = A.leafNode[jl];
+    const double c[3] = {c0, c1, c2}, s[3] = {s0, s1, s2};
+    const double cj[3] = {A.cx[node], A.cy[node], A.cz[node]}, sj[3] = {A.sx[node], A.sy[node], A.sz[node]};
+    double d2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        double dx = cj[a] - c[a];
+        if constexpr (PER) dx -= box.l[a] * rint(dx * box.il[a]);
+        const double v = fmax(fabs(dx) - s[a] - sj[a] - 2.0 * kFinestCell * box.l[a], 0.0);
+        d2 += v * v;
+    }
+    return d2;
+}
+
+// Plain pair term (the reference's arithmetic): sums W and -q dW/dq - 3W when q < R.
+template<bool PER>
+__device__ inline void pairPlain(double dx, double dy, double dz, const DBox& box, double hs, double& sw, double& sg,
+                                 int& acc)
+{
+    imageFP64<PER>(dx, dy, dz, box);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double q2 = r2 * hs;
+    if (q2 < kRad2)
+    {
+        const double q = sqrt(q2);
+        double wk, dwk;
+        kernWdW<double>(q2, q, wk, dwk);
+        sw += wk;
+        sg += -q * dwk - 3.0 * wk;
+        ++acc;
+    }
+}
+
+// Mixed pair term: FP32 separation (exact to one rounding), r^2, support test, q and kernel
+// brackets V, D (kernVD); V and q D summed in FP64 when q < R.
+// q^2 = r^2 (hsr + hsrLo), hsr + hsrLo = 1/h^2 in units of the separation's.
+__device__ inline void pairMixed(float dx, float dy, float dz, float hsr, float hsrLo, double& sv, double& sp,
+                                 int& acc)
+{
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    float q2 = r2 * hsr;
+    if (q2 < float(kRad2))
+    {
+        q2 = fmaf(r2, hsrLo, q2);
+        const float q = __builtin_amdgcn_sqrtf(q2);
+        float V, D;
+        kernVD<float>(q, V, D);
+        sv += (double)V;
+        sp += (double)(q * D);
+        ++acc;
+    }
+}
+
+// Particle sums from the accumulators: (sum W, sum -q dW/dq - 3W).
+template<bool MIXED>
+__device__ inline void particleSums(double s0, double s1, double& wsum, double& gsum)
+{
+    if constexpr (MIXED)
+    {
+        wsum = kWw * s0;
+        gsum = kGp * s1 - kGw * s0;
+    }
+    else
+    {
+        wsum = s0;
+        gsum = s1;
+    }
+}
+
+// PP: the launch holds per-particle items (a separate instance keeps the register budget of
+// the whole-leaf path low when it does not).
+template<class Real, bool PER, int MODE, bool PP>
+__global__ __launch_bounds__(kNrThreads) void densityKernel(DensArgs A)
+{
+    constexpr bool kMixed = sizeof(Real) == 4;
+    __shared__ int segStart[kNrWaves][kWave];
+    __shared__ int segJ[kNrWaves][kWave];
+    __shared__ int rankLane[kNrWaves][kWave];   // whole-leaf items: k-th particle to evaluate -> lane
+    // whole-leaf tiles: mixed = fixed-point offsets (x, y, z, pad), one 16-byte LDS read per j;
+    // plain = FP64 positions
+    __shared__ int4   tq[kMixed ? kNrWaves : 1][kMixed ? 2 * kWave : 1];
+    __shared__ double tx[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
+    __shared__ double ty[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
+    __shared__ double tz[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
+
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = laneId();
+    const int item = blockIdx.x * kNrWaves + w;
+    unsigned long long ctr[4] = {0, 0, 0, 0};   // NR pairs, NR candidates, closure pairs, closure candidates
+
+    if (item < A.nItems)
+    {
+        const int a = PP ? A.itemPos[item] : item;
+        const int part = PP ? __builtin_amdgcn_readfirstlane(A.itemPart[item]) : -1;
+        const int iLeaf = A.leaves ? A.leaves[a] : a;
+        const uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
+        const DBox box = A.boxes[A.leafSet[iLeaf]];
+
+        if (PP && part >= 0)
+        {
+            // ---- one particle, 64 lanes over its candidates ----
+            const uint32_t i = (uint32_t)part;
+            const uint8_t st = MODE == 2 ? 0 : A.state[i];
+            const double xi = A.x[i], yi = A.y[i], zi = A.z[i], hcur = A.h[i];
+            const double hsi = 1.0 / (hcur * hcur);
+            const float hsr = float(hsi);
+            const float hsrLo = float(hsi - (double)hsr);
+            const double reach2 = (kRad * hcur) * (kRad * hcur) * kPruneSlack;
+            double s0 = 0, s1 = 0;
+            int acc = 0;
+            uint32_t cand = 0;
+            auto keep = [&](int jl) { return gapToLeaf<PER>(A, box, xi, yi, zi, 0.0, 0.0, 0.0, jl) <= reach2; };
+            auto pair = [&](int, int j) {
+                double dx = xi - A.x[j], dy = yi - A.y[j], dz = zi - A.z[j];
+                if constexpr (kMixed)
+                {
+                    imageFP64<PER>(dx, dy, dz, box);
+                    pairMixed(float(dx), float(dy), float(dz), hsr, hsrLo, s0, s1, acc);
+                }
+                else
+                {
+                    pairPlain<PER>(dx, dy, dz, box, hsi, s0, s1, acc);
+                }
+                ++cand;
+            };
+            forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, pair, [](int) {}, [](int) {});
+            s0 = waveReduceSum(s0);
+            s1 = waveReduceSum(s1);
+            const int slot = isClosureEval<MODE>(st) ? 2 : 0;
+            ctr[slot] = (unsigned long long)acc;
+            ctr[slot + 1] = cand;
+            double wsum, gsum;
+            particleSums<kMixed>(s0, s1, wsum, gsum);
+            if (lane == 0) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
+        }
+        else
+        {
+            // ---- whole leaf: its particles still to evaluate, up to 64 at a time, each with
+            // g = floor(64 / count) lanes splitting every tile ----
+            const uint32_t p0 = A.layout[iLeaf], p1 = A.layout[iLeaf + 1];
+            const int node = A.leafNode[iLeaf];
+            const double c0 = A.cx[node], c1 = A.cy[node], c2 = A.cz[node];
+            const double s0n = A.sx[node], s1n = A.sy[node], s2n = A.sz[node];
+            double hmList = A.hmaxLeaf[iLeaf];   // bounds the h of every particle evaluated here
+            if (!isfinite(hmList) || hmList <= 0.0) hmList = 0.0;
+            // mixed: offsets from the i-leaf centre in units of range / 2^29, range = leaf
+            // half-size + R hmax (bounds every i offset); integer differences are exact, so a
+            // pair separation is rounded once, to FP32, relative to itself.  j offsets are
+            // clamped to +-2^30: a clamped j is beyond R h_i of every i, differences fit int32.
+            double S = 0.0, invS2 = 0.0;
+            if constexpr (kMixed)
+            {
+                const double range = fmax(fmax(s0n, s1n), s2n) + kRad * hmList;
+                S = 536870912.0 / range;   // 2^29
+                invS2 = 1.0 / (S * S);
+            }
+            auto toFixed = [&](double v) -> int {
+                v = rint(v * S);
+                v = fmin(fmax(v, -1073741824.0), 1073741824.0);   // +-2^30
+                return (int)v;
+            };
+            if constexpr (kMixed) tq[w][kWave + lane] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);
+            else { tx[w][kWave + lane] = 1e200; ty[w][kWave + lane] = 1e200; tz[w][kWave + lane] = 1e200; }
+            for (uint32_t q0 = p0; q0 < p1; q0 += kWave)
+            {
+                const uint32_t ic = q0 + lane;
+                const uint8_t stc = (ic < p1 && MODE != 2) ? A.state[ic] : 0;
+                const bool want = ic < p1 && (MODE != 0 || stc != 0);
+                const uint64_t mask = ballot64(want);
+                if (mask == 0) continue;
+                const int c = __builtin_amdgcn_readfirstlane(__popcll(mask));
+                const int g = __builtin_amdgcn_readfirstlane(kWave / c);   // lanes per particle
+                if (want) rankLane[w][__popcll(mask & lanemaskLt())] = lane;
+                waveLdsSync();
+                const int grp = lane / g, sub = lane - grp * g;
+                const bool has = grp < c;
+                const uint32_t i = has ? q0 + (uint32_t)rankLane[w][grp] : p0;
+                const uint8_t st = (has && MODE != 2) ? A.state[i] : 0;
+                const double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+                const double hcur = has ? A.h[i] : 1.0;
+                const double hsi = 1.0 / (hcur * hcur);
+                int xq = 0, yq = 0, zq = 0;
+                if constexpr (kMixed) { xq = toFixed(xi - c0); yq = toFixed(yi - c1); zq = toFixed(zi - c2); }
+                const double hsd = hsi * invS2;
+                const float hsr = float(hsd);
+                const float hsrLo = float(hsd - (double)hsr);
+                double s0 = 0, s1 = 0;
+                int acc = 0;
+                uint32_t cand = 0;
+                auto load = [&](int l, int j) {
+                    double xj = A.x[j], yj = A.y[j], zj = A.z[j];
+                    if constexpr (kMixed)
+                    {
+                        xj -= c0; yj -= c1; zj -= c2;
+                        imageFP64<PER>(xj, yj, zj, box);
+                        tq[w][l] = make_int4(toFixed(xj), toFixed(yj), toFixed(zj), 0);
+                    }
+                    else
+                    {
+                        tx[w][l] = xj; ty[w][l] = yj; tz[w][l] = zj;
+                    }
+                };
+                // slots past the tile's end hold a j beyond every support: 2^30 fixed-point
+                // units (>= 2 range) from the centre, or 1e200 in FP64 (r^2 overflows to inf)
+                auto pad = [&](int l) {
+                    if constexpr (kMixed) tq[w][l] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);
+                    else { tx[w][l] = 1e200; ty[w][l] = 1e200; tz[w][l] = 1e200; }
+                };
+                // pairs with tile slots first, first + stride, ...; each slot is read one step
+                // ahead.  Slots from nt on hold padding: [nt, 64) written with the tile, the
+                // upper half [64, 128) once per wavefront; a group of g lanes reads at most
+                // slot g - 1 + ceil(nt / g) g < 128.
+                auto sweep = [&](int first, int stride, int steps) {
+                    if constexpr (kMixed)
+                    {
+                        const int4* ptr = &tq[w][first];
+                        int4 nx = *ptr;
+#pragma unroll 4
+                        for (int t = 0; t < steps; ++t)
+                        {
+                            const int4 q = nx;
+                            ptr += stride;
+                            nx = *ptr;
+                            pairMixed((float)(xq - q.x), (float)(yq - q.y), (float)(zq - q.z), hsr, hsrLo, s0, s1,
+                                      acc);
+                        }
+                    }
+                    else
+                    {
+#pragma unroll 4
+                        for (int t = 0; t < steps; ++t)
+                        {
+                            const int k = first + t * stride;
+                            pairPlain<PER>(xi - tx[w][k], yi - ty[w][k], zi - tz[w][k], box, hsi, s0, s1, acc);
+                        }
+                    }
+                };
+                auto consume = [&](int nt) {
+                    if (g == 1)
+                    {
+                        sweep(0, 1, nt);   // list order, as the reference
+                    }
+                    else
+                    {
+                        sweep(sub, g, __builtin_amdgcn_readfirstlane((nt + g - 1) / g));
+                    }
+                    cand += sub == 0 ? (uint32_t)nt : 0u;
+                };
+                forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, [](int) { return true; }, load, pad,
+                              consume);
+                if (has)
+                {
+                    const int slot = isClosureEval<MODE>(st) ? 2 : 0;
+                    ctr[slot] += (unsigned long long)acc;
+                    ctr[slot + 1] += cand;
+                }
+                // group sums, a tree over lanes sub .. g-1 of each group (lane sub = 0 ends with it)
+                for (int off = 1; off < g; off <<= 1)
+                {
+                    const double t0 = __shfl_down(s0, off, kWave), t1 = __shfl_down(s1, off, kWave);
+                    if (sub + off < g) { s0 += t0; s1 += t1; }
+                }
+                double wsum, gsum;
+                particleSums<kMixed>(s0, s1, wsum, gsum);
+                if (has && sub == 0) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
+            }
+        }
+    }
+    blockCounters(A.counters, ctr);
+}
+
+// ---------------------------------------------------------------------------------------
+// Compaction and work items
+// ---------------------------------------------------------------------------------------
+__device__ inline uint32_t itemsOf(uint32_t remaining, int sparse)
+{
+    return remaining == 0 ? 0u : (remaining <= (uint32_t)sparse ? remaining : 1u);
+}
+
+// Wave per active leaf: flag[a] = leaf a has a particle in state != 0, count[a] = its work
+// items in the next pass; counters: particles in state 1, in state != 0, items.
+__global__ __launch_bounds__(256) void leafFlagKernel(const int* __restrict__ leaves, int nLeaves,
+                                                      const uint32_t* __restrict__ layout,
+                                                      const uint8_t* __restrict__ state, int sparse,
+                                                      uint32_t* __restrict__ flag, uint32_t* __restrict__ count,
+                                                      unsigned long long* __restrict__ counters)
+{
+    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= nLeaves) return;
+    const int l = leaves[a];
+    const uint32_t p0 = layout[l], p1 = layout[l + 1];
+    uint32_t it = 0, rem = 0;
+    for (uint32_t p = p0 + laneId(); p < p1; p += kWave)
+    {
+        const uint8_t s = state[p];
+        it += (s == 1) ? 1u : 0u;
+        rem += s ? 1u : 0u;
+    }
+    it = waveReduceSum(it);
+    rem = waveReduceSum(rem);
+    if (laneId() == 0)
+    {
+        const uint32_t items = itemsOf(rem, sparse);
+        flag[a] = rem ? 1u : 0u;
+        count[a] = items;
+        unsigned long long* slot = counters + (blockIdx.x % kCounterSlots) * kCounterStride;
+        if (it) atomicAdd(slot + kCtrIterating, (unsigned long long)it);
+        if (rem) atomicAdd(slot + kCtrRemaining, (unsigned long long)rem);
+        if (items) atomicAdd(slot + kCtrItems, (unsigned long long)items);
+    }
+}
+
+__global__ void leafScatterKernel(const int* __restrict__ leaves, const uint32_t* __restrict__ count, int nLeaves,
+                                  const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                  int* __restrict__ out, uint32_t* __restrict__ outCount)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < nLeaves && flag[a])
+    {
+        out[pos[a]] = leaves[a];
+        outCount[pos[a]] = count[a];
+    }
+}
+
+// Wave per active leaf: its items at off[a]: (a, -1) for a whole-leaf item, else (a, p) for
+// every particle p in state != 0, in particle order.  The rule matches leafFlagKernel's count.
+__global__ __launch_bounds__(256) void itemFillKernel(const int* __restrict__ leaves, int nLeaves,
+                                                      const uint32_t* __restrict__ layout,
+                                                      const uint8_t* __restrict__ state, int sparse,
+                                                      const uint32_t* __restrict__ off, int* __restrict__ itemPos,
+                                                      int* __restrict__ itemPart)
+{
+    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= nLeaves) return;
+    const int l = leaves[a];
+    const uint32_t p0 = layout[l], p1 = layout[l + 1];
+    uint32_t rem = 0;
+    for (uint32_t p = p0 + laneId(); p < p1; p += kWave) rem += state[p] ? 1u : 0u;
+    rem = waveReduceSum(rem);
+    if (rem == 0) return;
+    uint32_t o = off[a];
+    if (rem > (uint32_t)sparse)
+    {
+        if (laneId() == 0) { itemPos[o] = a; itemPart[o] = -1; }
+        return;
+    }
+    for (uint32_t q0 = p0; q0 < p1; q0 += kWave)
+    {
+        const uint32_t p = q0 + laneId();
+        const bool live = p < p1 && state[p] != 0;
+        const uint64_t m = ballot64(live);
+        if (live)
+        {
+            const uint32_t k = o + (uint32_t)__popcll(m & lanemaskLt());
+            itemPos[k] = a;
+            itemPart[k] = (int)p;
+        }
+        o += (uint32_t)__popcll(m);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Host wrappers
+// ---------------------------------------------------------------------------------------
+void launchDensity(const DensArgs& a, int precision, int mode, bool periodic, hipStream_t st)
+{
+    if (a.nItems <= 0) return;
+    const int grid = iceil(a.nItems, kNrWaves);
+    const bool pp = a.itemPart != nullptr;
+#define CSPH_DENS(R, P, M, Q) densityKernel<R, P, M, Q><<<grid, kNrThreads, 0, st>>>(a)
+#define CSPH_DENS_PP(R, P, M)                     \
+    do {                                          \
+        if (pp) CSPH_DENS(R, P, M, true);         \
+        else    CSPH_DENS(R, P, M, false);        \
+    } while (0)
+#define CSPH_DENS_MODE(R, P)                      \
+    do {                                          \
+        if (mode == 0)      CSPH_DENS_PP(R, P, 0); \
+        else if (mode == 1) CSPH_DENS(R, P, 1, false); \
+        else                CSPH_DENS(R, P, 2, false); \
+    } while (0)
+    if (precision == kMixed)
+    {
+        if (periodic) CSPH_DENS_MODE(float, true);
+        else          CSPH_DENS_MODE(float, false);
+    }
+    else
+    {
+        if (periodic) CSPH_DENS_MODE(double, true);
+        else          CSPH_DENS_MODE(double, false);
+    }
+#undef CSPH_DENS_MODE
+#undef CSPH_DENS_PP
+#undef CSPH_DENS
+    LAUNCH_CHECK();
+}
+
+int counterSlots() { return kCounterSlots; }
+int counterStride() { return kCounterStride; }
+
+void launchLeafFlags(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* state, int sparse,
+                     uint32_t* flag, uint32_t* itemCount, unsigned long long* counters, hipStream_t st)
+{
+    if (nLeaves <= 0) return;
+    leafFlagKernel<<<iceil(nLeaves, 4), 256, 0, st>>>(leaves, nLeaves, layout, state, sparse, flag, itemCount,
+                                                      counters);
+    LAUNCH_CHECK();
+}
+
+void launchLeafScatter(const int* leaves, const uint32_t* count, int nLeaves, const uint32_t* flag,
+                       const uint32_t* pos, int* out, uint32_t* outCount, hipStream_t st)
+{
+    if (nLeaves <= 0) return;
+    leafScatterKernel<<<iceil(nLeaves, 256), 256, 0, st>>>(leaves, count, nLeaves, flag, pos, out, outCount);
+    LAUNCH_CHECK();
+}
+
+void launchItemFill(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* state, int sparse,
+                    const uint32_t* off, int* itemPos, int* itemPart, hipStream_t st)
+{
+    if (nLeaves <= 0) return;
+    itemFillKernel<<<iceil(nLeaves, 4), 256, 0, st>>>(leaves, nLeaves, layout, state, sparse, off, itemPos, itemPart);
+    LAUNCH_CHECK();
+}
+
+} // namespace csph

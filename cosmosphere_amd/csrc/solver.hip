@@ -10,8 +10,12 @@
  * Differences of form, not of result: lists are built for the active leaves only (the
  * reference builds all leaves and reads only the active ones; tree.cuh:455-491), and the
  * active set is kept as a per-particle flag plus a compacted leaf list rather than
- * copy_if + sort + unique (density_base.cu:856-886).  The host reads two scalars per
- * iteration (list total, active-leaf count).
+ * copy_if + sort + unique (density_base.cu:856-886).  The host reads the list total and
+ * the counters (active leaves, particles still iterating, work items) once per pass.
+ *
+ * Closure.  With the closure on, a converged particle is evaluated once more at its final h
+ * in the following pass (state 2), so the returned rho belongs to the returned h; passes that
+ * hold only closures are not counted as iterations.
  */
 #include "density.hpp"
 #include "solver.hpp"
@@ -80,17 +84,17 @@ static unsigned long long* zeroCounters(State& s)
     return c;
 }
 
-// Sums counter k over the slots (after the stream is synchronised).
-static void readCounters(State& s, unsigned long long out[3])
+// Sums every counter over the slots (after the stream is synchronised).
+static void readCounters(State& s, unsigned long long out[kNumCounters])
 {
     const int ns = counterSlots(), st = counterStride();
     std::vector<unsigned long long> h((size_t)ns * st);
     HIP_CHECK(hipMemcpyAsync(h.data(), s.counters.as<unsigned long long>(), h.size() * 8, hipMemcpyDeviceToHost,
                              s.stream));
     HIP_CHECK(hipStreamSynchronize(s.stream));
-    out[0] = out[1] = out[2] = 0;
+    for (int c = 0; c < kNumCounters; ++c) out[c] = 0;
     for (int k = 0; k < ns; ++k)
-        for (int c = 0; c < 3; ++c) out[c] += h[(size_t)k * st + c];
+        for (int c = 0; c < kNumCounters; ++c) out[c] += h[(size_t)k * st + c];
 }
 
 static bool periodicOf(const State& s) { return !s.hBox.empty() && s.hBox[0].periodic; }
@@ -113,12 +117,14 @@ static DensArgs densArgs(State& s, const int* leaves, int nLeaves, double pmass,
     a.h = s.h.as<double>();
     a.rho = s.rho.as<double>();
     a.gradh = s.gradh.as<double>();
-    a.active = s.active.as<uint8_t>();
+    a.state = s.active.as<uint8_t>();
     a.boxes = s.box.as<DBox>();
     a.pmass = pmass;
     a.tolh = tolh;
     a.hfact = hfact;
+    a.pmassSet = s.usePmassSet ? s.pmassSet.as<double>() : nullptr;
     a.counters = counters;
+    a.nItems = nLeaves;   // whole-leaf items unless the caller sets the item arrays
     return a;
 }
 
@@ -144,46 +150,99 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
     const int n = s.n, na = s.totalAlive, L = s.nLeaves;
     const bool per = periodicOf(s);
 
-    launchInitActive(s.active.as<uint8_t>(), na, n, s.rho.as<double>(), s.gradh.as<double>(), st);
+    uint8_t* state = s.active.as<uint8_t>();
+    const uint32_t* layout = s.layout.as<uint32_t>();
+    launchInitActive(state, na, n, s.rho.as<double>(), s.gradh.as<double>(), st);
     int* leaves = s.activeLeaves.get<int>(L + 1);
     int* leavesAlt = s.activeLeavesAlt.get<int>(L + 1);
     uint32_t* flag = s.leafFlag.get<uint32_t>(2 * (size_t)L + 4);
     uint32_t* pos = flag + L + 2;
+    uint32_t* cnt = s.itemCount.get<uint32_t>(L + 1);
+    uint32_t* cntAlt = s.itemCountAlt.get<uint32_t>(L + 1);
+    uint32_t* itemOff = s.itemOff.get<uint32_t>(L + 2);
+    int* itemPos = s.itemPos.get<int>((size_t)na + 1);
+    int* itemPart = s.itemPart.get<int>((size_t)na + 1);
+    // work-item rule: whole leaves only for WARP_PER_LEAF (every particle of an active leaf is
+    // recomputed) and for the whole-leaf mapping; per-particle items for every remaining
+    // particle with the flat mapping
+    const int sparse = (mode == 1 || s.mapping == kMapWavePerLeaf) ? 0
+                       : (s.mapping == kMapFlat ? (1 << 30) : std::max(s.sparse, 0));
+
+    // Compaction of the active leaves after a pass: flags and item counts, scan, scatter; the
+    // host reads the new leaf count and the counters (one synchronisation per pass).
+    unsigned long long c[kNumCounters];
+    int nAL = 0;
+    auto compact = [&](int nIn, unsigned long long* ctr, Ev* done) {
+        launchLeafFlags(leaves, nIn, layout, state, sparse, flag, cnt, ctr, st);
+        scanExclusive<uint32_t>(flag, pos, nIn, s.scanTmp, nullptr, st);
+        launchLeafScatter(leaves, cnt, nIn, flag, pos, leavesAlt, cntAlt, st);
+        if (done) done->rec(st);
+        unsigned int nNext = 0;
+        if (nIn > 0) HIP_CHECK(hipMemcpyAsync(&nNext, pos + nIn, sizeof(nNext), hipMemcpyDeviceToHost, st));
+        readCounters(s, c);   // synchronises
+        std::swap(leaves, leavesAlt);
+        std::swap(cnt, cntAlt);
+        std::swap(s.activeLeaves, s.activeLeavesAlt);
+        std::swap(s.itemCount, s.itemCountAlt);
+        nAL = (int)nNext;
+    };
+
+    s.passLog.clear();
     launchIota(leaves, L, st);
-    int nAL = na > 0 ? L : 0;
-    unsigned long long nActive = (unsigned long long)na;
+    compact(na > 0 ? L : 0, zeroCounters(s), nullptr);   // drops the leaves without live particles
+    unsigned long long nIterating = c[kCtrIterating], nItems = c[kCtrItems];
 
     Ev a0, a1, a2, a3;
-    for (int iter = 0; iter < s.maxIter && nAL > 0; ++iter)
+    while (nAL > 0)
     {
+        const bool nrPass = nIterating > 0;   // else: closures only
+        if (nrPass && t.iters >= s.maxIter) break;
+        // per-particle items once few leaves remain (the chip is then underfilled and long
+        // lists dominate); whole-leaf items with grouped lanes otherwise
+        const bool perParticle = sparse > 0 && (s.mapping == kMapFlat || nAL <= s.tailLeaves);
         a0.rec(st);
-        hmaxLeaves(s, leaves, nAL);
-        buildLists(s, leaves, nAL, false);
+        hmaxLeaves(s, leaves, nAL, mode == 0 ? state : nullptr);
+        buildLists(s, leaves, nAL, false, mode == 0);   // tight: reach of the particles still to evaluate
+        if (perParticle)
+        {
+            scanExclusive<uint32_t>(cnt, itemOff, nAL, s.scanTmp, nullptr, st);
+            launchItemFill(leaves, nAL, layout, state, sparse, itemOff, itemPos, itemPart, st);
+        }
         a1.rec(st);
         unsigned long long* ctr = zeroCounters(s);
-        launchDensity(densArgs(s, leaves, nAL, pmass, tolh, hfact, ctr), s.precision, mode, per, st);
+        DensArgs da = densArgs(s, leaves, nAL, pmass, tolh, hfact, ctr);
+        if (mode == 0) da.hmaxLeaf = s.hRemLeaf.as<double>();   // bounds every h evaluated in the pass
+        da.itemPos = perParticle ? itemPos : nullptr;
+        da.itemPart = perParticle ? itemPart : nullptr;
+        da.nItems = perParticle ? (int)nItems : nAL;
+        da.closure = closure && mode == 0 && !withGradients;   // the gradient sweep recomputes rho
+        launchDensity(da, s.precision, mode, per, st);
         a2.rec(st);
-        launchLeafFlags(leaves, nAL, s.layout.as<uint32_t>(), s.active.as<uint8_t>(), flag, ctr + 2, st);
-        scanExclusive<uint32_t>(flag, pos, nAL, s.scanTmp, nullptr, st);
-        launchLeafScatter(leaves, nAL, flag, pos, leavesAlt, st);
-        a3.rec(st);
-        unsigned int nNext = 0;
-        HIP_CHECK(hipMemcpyAsync(&nNext, pos + nAL, sizeof(nNext), hipMemcpyDeviceToHost, st));
-        unsigned long long c[3];
-        readCounters(s, c);   // synchronises
-        t.jbuild += secs(a0, a1);
-        t.nrkern += secs(a1, a2);
-        t.compact += secs(a2, a3);
-        t.pairsAccepted += (double)c[0];
-        t.pairsCandidate += (double)c[1];
+        const int nALpass = nAL;
+        compact(nAL, ctr, &a3);
+        const double tj = secs(a0, a1), tk = secs(a1, a2), tc = secs(a2, a3);
+        s.passLog.push_back(PassRecord{(double)nALpass, (double)da.nItems, (double)nIterating, (double)c[kCtrCand],
+                                       (double)c[kCtrPairs], (double)c[kCtrClosCand], tk, (double)perParticle});
+        if (nrPass)
+        {
+            t.jbuild += tj;
+            t.nrkern += tk;
+            t.compact += tc;
+            t.iters += 1;
+        }
+        else
+        {
+            t.closure += tj + tk + tc;
+        }
+        t.pairsAccepted += (double)c[kCtrPairs];
+        t.pairsCandidate += (double)c[kCtrCand];
+        t.closurePairs += (double)c[kCtrClosPairs];
+        t.closureCand += (double)c[kCtrClosCand];
         t.kernLaunches += 1;
-        nActive = c[2];
-        t.iters = iter + 1;
-        std::swap(leaves, leavesAlt);
-        std::swap(s.activeLeaves, s.activeLeavesAlt);
-        nAL = (int)nNext;
+        nIterating = c[kCtrIterating];
+        nItems = c[kCtrItems];
     }
-    if (na == 0) nActive = 0;
+    const unsigned long long nActive = na > 0 ? nIterating : 0;
 
     int hovf[2] = {0, 0};
     HIP_CHECK(hipMemcpyAsync(hovf, ovf, sizeof(hovf), hipMemcpyDeviceToHost, st));
@@ -216,22 +275,21 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         }
     }
 
-    // closure (mixed density-only form): rho and d(rho)/d(h) at the converged h, with FP64
-    // sums, so the returned rho belongs to the returned h
-    if (closure && !withGradients && L > 0 && na > 0)
+    // WARP_PER_LEAF with the closure: one all-leaf pass at the final h (the FLAT solve folds
+    // its closures into the iterations)
+    if (closure && mode == 1 && !withGradients && L > 0 && na > 0)
     {
         Ev c0, c1;
         c0.rec(st);
         hmaxLeaves(s, nullptr, L);
         buildLists(s, nullptr, L, false);
         unsigned long long* ctr = zeroCounters(s);
-        launchDensity(densArgs(s, nullptr, L, pmass, tolh, hfact, ctr), kPlainFP64, 2, per, st);
+        launchDensity(densArgs(s, nullptr, L, pmass, tolh, hfact, ctr), s.precision, 2, per, st);
         c1.rec(st);
-        unsigned long long c[3];
         readCounters(s, c);
-        t.closure = secs(c0, c1);
-        t.closurePairs = (double)c[0];
-        t.closureCand = (double)c[1];
+        t.closure += secs(c0, c1);
+        t.closurePairs += (double)c[kCtrClosPairs];
+        t.closureCand += (double)c[kCtrClosCand];
         s.listLeaves = L;
         s.listSymmetric = 0;
     }
@@ -329,7 +387,7 @@ ForceTimings computeForces(State& s, double pmass, double beta, double alphau, b
     launchDeadForce(na, n, s.fx.as<double>(), s.fy.as<double>(), s.fz.as<double>(), s.f4.as<double>(),
                     s.vsig.as<double>(), s.divvF.as<double>(), st);
     e3.rec(st);
-    unsigned long long c[3];
+    unsigned long long c[kNumCounters];
     readCounters(s, c);
     ft.upsweep = 0.0;
     ft.jbuild = secs(e0, e1);

@@ -53,6 +53,13 @@ struct Timings
     double closurePairs = 0, closureCand = 0;
 };
 
+// One NR pass: active leaves, work items, particles iterating at its start, NR candidates and
+// accepted pairs, closure candidates, kernel seconds, 1 if per-particle items were used.
+struct PassRecord
+{
+    double leaves, items, iterating, candidates, pairs, closureCandidates, kernelSeconds, perParticle;
+};
+
 struct ForceTimings
 {
     double upsweep = 0, jbuild = 0, prep = 0, kernel = 0, total = 0;
@@ -86,6 +93,8 @@ struct State
     std::vector<int> hLeafFirst, hLeafN, hIntFirst, hIntN, hNodeFirst;
     std::vector<DBox> hBox;
     DevBuf setFirst, setN, aliveFirst, alive, leafFirst, intFirst, nodeFirst, box;
+    DevBuf pmassSet;          // ensemble: particle mass of each set
+    int usePmassSet = 0;
     int totalAlive = 0;
 
     // ---- sort / tree scratch ----
@@ -115,9 +124,10 @@ struct State
 
     // ---- NR and lists ----
     DevBuf hmaxLeaf;
+    DevBuf hRemLeaf;     // max h of the particles still to evaluate (tight NR lists)
     DevBuf activeLeaves, activeLeavesAlt, leafFlag;
-    DevBuf active;       // u8 per particle: still iterating
-    DevBuf converged;    // u8 per particle
+    DevBuf active;       // u8 per particle: 1 iterating, 2 closure pending, 0 done
+    DevBuf itemCount, itemCountAlt, itemOff, itemPos, itemPart;   // NR work items
     DevBuf jCount, jOffset, jList;
     int    listLeaves = 0;    // number of leaves the CSR lists cover (all-leaf lists)
     int    listTotal  = 0;
@@ -135,7 +145,10 @@ struct State
     uint64_t token = 0, jlistToken = 0;
     int      maxIter = 100;
     int      precision = kPlainFP64;
-    int      mapping = kMapAuto;
+    int      mapping = kMapAuto;   // auto: whole-leaf items, per-particle below `sparse`
+    int      sparse = 16;          // per-particle items when at most this many remain in a leaf
+    int      tailLeaves = 2048;    // ... and at most this many leaves are active
+    std::vector<PassRecord> passLog;   // one record per NR pass of the last solve
     int      closureOn = -1;   // -1: on for mixed, off for plain
     int      treeOnly = 0;     // verification: stop after the tree build
     double   uploadSeconds = 0.0, downloadSeconds = 0.0;

@@ -12,12 +12,13 @@ namespace csph
 constexpr int kCounterSlots  = 256;
 constexpr int kCounterStride = 16;   // u64 per slot: one 128-byte line each
 
-// Stage the j-particles of list entries [lo, hi) as 64-slot tiles.  For every tile, each
-// lane with a slot calls load(slot, j); then consume(nt) runs with the tile complete.
-template<class Load, class Consume>
-__device__ inline void forEachTile(int* __restrict__ segStart, int* __restrict__ segJ,
-                                   const uint32_t* __restrict__ layout, const int* __restrict__ jList, uint32_t lo,
-                                   uint32_t hi, Load load, Consume consume)
+// Stage the j-particles of list entries [lo, hi) as 64-slot tiles, skipping the entries
+// whose leaf keep(jl) rejects.  For every tile, each lane with a slot calls load(slot, j) and
+// each lane without one pad(slot); then consume(nt) runs with the tile complete.
+template<class Keep, class Load, class Pad, class Consume>
+__device__ inline void forEachTileIf(int* __restrict__ segStart, int* __restrict__ segJ,
+                                     const uint32_t* __restrict__ layout, const int* __restrict__ jList, uint32_t lo,
+                                     uint32_t hi, Keep keep, Load load, Pad pad, Consume consume)
 {
     const int lane = laneId();
     for (uint32_t e0 = lo; e0 < hi; e0 += kWave)
@@ -27,8 +28,11 @@ __device__ inline void forEachTile(int* __restrict__ segStart, int* __restrict__
         if (e < hi)
         {
             const int jl = jList[e];
-            js = (int)layout[jl];
-            jc = (int)layout[jl + 1] - js;
+            if (keep(jl))
+            {
+                js = (int)layout[jl];
+                jc = (int)layout[jl + 1] - js;
+            }
         }
         const int incl  = waveInclusiveScan(jc);
         const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));   // wave-uniform
@@ -50,11 +54,23 @@ __device__ inline void forEachTile(int* __restrict__ segStart, int* __restrict__
                 }
                 load(lane, segJ[a] + slot - segStart[a]);
             }
+            else
+            {
+                pad(lane);
+            }
             waveLdsSync();
             consume(__builtin_amdgcn_readfirstlane(min(kWave, total - s0)));
             waveLdsSync();
         }
     }
+}
+
+template<class Load, class Consume>
+__device__ inline void forEachTile(int* __restrict__ segStart, int* __restrict__ segJ,
+                                   const uint32_t* __restrict__ layout, const int* __restrict__ jList, uint32_t lo,
+                                   uint32_t hi, Load load, Consume consume)
+{
+    forEachTileIf(segStart, segJ, layout, jList, lo, hi, [](int) { return true; }, load, [](int) {}, consume);
 }
 
 } // namespace csph

@@ -91,7 +91,10 @@ int cosmo_ext_solve_dens_h(int n, double pmass, int mode, int withGradients, int
                            double tolh, double hfact);
 
 /* Options: "precision" (0 = plain FP64, 1 = mixed FP32-offset form), "max_iter",
- * "mapping" (0 auto, 1 wave per leaf, 2 flat), "closure" (-1 auto, 0 off, 1 on),
+ * "mapping" (NR work items: 0 auto = one wavefront per leaf, one per particle once at most
+ * "sparse" particles of the leaf remain; 1 = always one per leaf; 2 = always one per
+ * particle), "sparse" (default 16), "closure" (-1 auto = on for mixed, 0 off, 1 on: rho and
+ * d(rho)/d(h) re-evaluated at the converged h),
  * "tree_only" (1: stop after the tree build, for verification of the tree stages).
  * The environment variable COSMO_PRECISION=mixed|fp64 sets the precision for
  * densityiterate_gpu_c. */
@@ -109,6 +112,16 @@ long long cosmo_ext_get_array(const char* name, void* dst, long long maxBytes);
 void cosmo_ext_stash_inputs(void);
 void cosmo_ext_restore_inputs(void);
 void cosmo_ext_sync(void);
+
+/* Ensemble of independent particle sets solved together (one sequence of kernel launches for
+ * all sets; each set has its own root box, tree, particle mass and convergence).  The sets lie
+ * one after another in every slot: set b owns entries [counts[0] + ... + counts[b-1],
+ * + counts[b]) of the caller's order.  cosmo_batch_init sizes the device arrays for the total;
+ * upload and download then use the slot calls with n = total.  cosmo_batch_solve runs the
+ * density-only Newton-Raphson solve (FLAT_PARTICLE) with pmass[b] for set b; boxes holds six
+ * doubles per set when periodic != 0.  Returns the Newton iterations run. */
+void cosmo_batch_init(int nSets, const int* counts);
+int cosmo_batch_solve(const double* pmass, int periodic, const double* boxes, double tolh, double hfact);
 
 #ifdef __cplusplus
 }

@@ -39,11 +39,17 @@ def dataset(name):
     raise KeyError(name)
 
 
-def gpu_run(c, precision=0, gradients=True, mode=0, max_iter=100, v=None, a=None, tree_only=False):
+DEFAULT_ITEMS = {"mapping": 0, "sparse": 16, "closure": -1}
+
+
+def gpu_run(c, precision=0, gradients=True, mode=0, max_iter=100, v=None, a=None, tree_only=False, **options):
+    """Solve set c through the C ABI; `options` override the work-item and closure defaults."""
     lib = Library(c["kernel"])
     lib.set_option("precision", precision)
     lib.set_option("max_iter", max_iter)
     lib.set_option("tree_only", int(tree_only))
+    for k, val in dict(DEFAULT_ITEMS, **options).items():
+        lib.set_option(k, val)
     n = c["n"]
     lib.arrays_init(n)
     lib.upload(1, np.stack([c["x"], c["y"], c["z"]]))

@@ -1,0 +1,77 @@
+"""GPU parity of the ensemble solve (cosmo_batch_*): independent sets of different kinds and
+sizes solved in one sequence of launches must each equal the oracle's solve of that set alone
+-- tree bit-exact per set, h and rho to the single-set tolerances."""
+import numpy as np
+import pytest
+
+from cosmosphere_amd import Library, workloads
+from cosmosphere_amd.api import DENS_OUT, HSML, POS
+from gpu_common import rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def ensemble():
+    sets = []
+    for b, n in enumerate((3000, 1700, 5000, 64, 2500)):
+        if b % 2 == 0:
+            x, y, z, h = workloads._jittered_lattice(n, 100 + b)
+            sets.append(workloads._pack(x, y, z, h, "cubic", "lattice %d" % b))
+        else:
+            x, y, z = workloads.plummer_mixture(n, 100 + b)
+            sets.append(workloads._pack(x, y, z, np.full(n, 1.2 * n ** (-1 / 3)), "cubic", "plummer %d" % b))
+    # a dead particle in one set
+    sets[2]["h"] = sets[2]["h"].copy()
+    sets[2]["h"][17] = -1.0
+    return sets
+
+
+def run_batch(sets, precision):
+    lib = Library("cubic")
+    lib.set_option("precision", precision)
+    lib.set_option("max_iter", 100)
+    lib.set_option("tree_only", 0)
+    counts = [s["n"] for s in sets]
+    n = lib.batch_init(counts)
+    lib.upload(POS, np.concatenate([np.stack([s["x"], s["y"], s["z"]]) for s in sets], axis=1))
+    lib.upload(HSML, np.concatenate([s["h"] for s in sets]))
+    lib.upload(3, np.zeros((3, n)))
+    lib.upload(4, np.zeros((3, n)))
+    iters = lib.batch_solve([s["pmass"] for s in sets], tolh=1e-4, hfact=1.2)
+    return lib, iters, n
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_matches_single_set_solves(precision):
+    sets = ensemble()
+    lib, iters, n = run_batch(sets, precision)
+    h = lib.download(HSML, n)[0]
+    d = lib.download(DENS_OUT, n)
+    leaf_set = lib.array("leafSet")
+    leaf_start = lib.array("leafStart")
+    first = 0
+    max_iters = 0
+    for b, s in enumerate(sets):
+        o = O.Oracle(quintic=False)
+        o.set_particles(s["x"], s["y"], s["z"], s["h"])
+        o.build()
+        assert o.density_solve(s["pmass"], 1e-4, 1.2, 100, 0, False) == 0
+        max_iters = max(max_iters, o.geti("itersRun"))
+        np.testing.assert_array_equal(leaf_start[leaf_set == b], o.get("csTree")[:-1])
+        order = o.get("order")
+        na = o.geti("nAlive")
+        h_ref = np.empty(s["n"])
+        h_ref[order] = o.get("h")
+        live = np.zeros(s["n"], bool)
+        live[order[:na]] = True
+        hb = h[first:first + s["n"]]
+        tol = 1e-10 if precision == 0 else 1e-6
+        assert rel_err(hb[live], h_ref[live]) < tol, (b, rel_err(hb[live], h_ref[live]))
+        np.testing.assert_array_equal(hb[~live], s["h"][~live])      # dead particles keep their h
+        if precision == 0:
+            rho_ref = np.empty(s["n"])
+            rho_ref[order] = o.get("rho")
+            assert rel_err(d[0][first:first + s["n"]][live], rho_ref[live]) < 1e-10
+        first += s["n"]
+    assert iters == max_iters

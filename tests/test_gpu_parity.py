@@ -4,7 +4,8 @@ Bit-exact: keys, sort permutation, live count, leaf tree (csTree, levels, layout
 internal nodes, node geometry, particle -> leaf map, j-leaf lists.
 Plain FP64 density solve: h, rho, d(rho)/d(h) within 1e-10 relative (summation order and
 fused multiply-add are the only differences), same iteration count.
-Mixed FP32-offset form: h within 1e-6 relative of the FP64 oracle, rho(h) closure within 1e-6.
+Mixed FP32-offset form: h and rho within 1e-6 relative of the FP64 oracle; rho within 5e-7 of
+the oracle's closure at the GPU's own h (the mixed closure sums FP32 pair terms in FP64).
 """
 import numpy as np
 import pytest
@@ -128,12 +129,12 @@ def test_density_mixed_vs_fp64_oracle(name):
     # rho of the independent FP64 solve, at its own converged h (its closure)
     o.closure(c["pmass"])
     er_solve = rel_err(rho, o.get("rho")[:na])
-    # rho at the GPU's h: the FP64 closure of the mixed form against the oracle's
+    # rho at the GPU's h: the mixed closure against the oracle's FP64 one
     o.set_h_sorted(lib.array("h"))
     o.closure(c["pmass"])
     er_same_h = rel_err(rho, o.get("rho")[:na])
     print("mixed", name, "iters", iters, o.geti("itersRun"),
           "max rel err h %.3e, rho vs oracle solve %.3e, rho at same h %.3e" % (eh, er_solve, er_same_h))
-    assert eh < 1e-6                  # north_star tolerance on h
-    assert er_same_h < 1e-12          # the closure is FP64
-    assert er_solve < 3.0 * 1e-6      # rho ~ h^-3: the 1e-6 budget on h is 3e-6 on rho(h)
+    assert eh < 1e-6                  # tolerance of the mixed form on h
+    assert er_same_h < 5e-7           # FP32 pair terms, FP64 sums and rho(h) formula
+    assert er_solve < 1e-6            # tolerance of the mixed form on rho
