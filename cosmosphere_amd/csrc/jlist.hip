@@ -16,14 +16,15 @@
  * The accept tests are evaluated exactly as written (no contraction), which keeps the
  * lists bit-identical to the FP64 oracle's.
  *
- * Tight lists (Newton-Raphson passes).  An i-leaf whose particles still to be evaluated all
- * have h below its hmax needs only the j-leaves within R max(h_remaining) of its box.  A
- * second test drops the nodes beyond that reach (Euclidean box gap, both boxes widened by
- * one finest cell: a particle lies within one finest cell of its node's box, since the key
- * quantisation floor(x m) - lo m truncates up to one cell low; relative slack 1e-5).  A node
- * it drops holds no pair inside any remaining particle's support, and a dropped internal
- * node's leaves are dropped with it, so the tight list is the reference list minus leaves
- * without such pairs, in the same depth-first order: the accepted pairs and their order are
+ * Tight lists (Newton-Raphson passes).  The reference's test accepts a node that overlaps
+ * the i-leaf box grown by R hmax along each axis: a cube around the support spheres, whose
+ * corners hold no pair.  A tight list adds a second test: the Euclidean gap between the two
+ * boxes (each widened by one finest cell: a particle lies within one finest cell of its
+ * node's box, since the key quantisation floor(x m) - lo m truncates up to one cell low;
+ * relative slack 1e-5) within R max(h) of the particles still to be evaluated.  A node it
+ * drops holds no pair inside any such particle's support, and a dropped internal node's
+ * leaves are dropped with it, so the tight list is the reference list minus leaves without
+ * such pairs, in the same depth-first order: the accepted pairs and their order are
  * unchanged.
  */
 #include "tile.hpp"
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
     if (!SYM && A.hRemLeaf)
     {
         const double hr = A.hRemLeaf[iLeaf];
-        if (hr < hmax) tight2 = (kRad * hr) * (kRad * hr) * (1.0 + 1e-5);
+        tight2 = (kRad * hr) * (kRad * hr) * (1.0 + 1e-5);
     }
     const double pad[3] = {2.0 * kFinestCell * box.l[0], 2.0 * kFinestCell * box.l[1], 2.0 * kFinestCell * box.l[2]};
 

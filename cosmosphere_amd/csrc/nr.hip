@@ -36,9 +36,10 @@
  *          order, per-particle items by a wavefront tree;
  *   mixed  pair separations are formed exactly and rounded once to FP32 -- 32-bit fixed-point
  *          offsets from the i-leaf centre in whole-leaf items, an FP64 difference in
- *          per-particle items -- then r^2, q, the kernel and the square root in FP32; each pair
- *          term is added to an FP64 accumulator; the NR update, the convergence test and the
- *          closure's rho(h) formula are FP64.
+ *          per-particle items -- then r^2, q, the square root and the kernel brackets in FP32;
+ *          the W bracket of every pair is added to an FP64 accumulator, the d(rho)/d(h) bracket
+ *          to an FP32 partial flushed into FP64 once per tile; the NR update, the convergence
+ *          test and the closure's rho(h) formula are FP64.
  */
 #include "density.hpp"
 #include "prims.hpp"
@@ -185,9 +186,11 @@ __device__ inline void pairPlain(double dx, double dy, double dz, const DBox& bo
 }
 
 // Mixed pair term: FP32 separation (exact to one rounding), r^2, support test, q and kernel
-// brackets V, D (kernVD); V and q D summed in FP64 when q < R.
+// brackets V, D (kernVD), when q < R.  V is summed in FP64 (it fixes rho(h), hence the root h);
+// q D in the caller's FP32 partial (it sets d(rho)/d(h), which enters only the Newton step
+// length and the d(rho)/d(h) output), which the caller adds into FP64 every <= 64 pairs.
 // q^2 = r^2 (hsr + hsrLo), hsr + hsrLo = 1/h^2 in units of the separation's.
-__device__ inline void pairMixed(float dx, float dy, float dz, float hsr, float hsrLo, double& sv, double& sp,
+__device__ inline void pairMixed(float dx, float dy, float dz, float hsr, float hsrLo, double& sv, float& sp,
                                  int& acc)
 {
     const float r2 = dx * dx + dy * dy + dz * dz;
@@ -199,7 +202,7 @@ __device__ inline void pairMixed(float dx, float dy, float dz, float hsr, float 
         float V, D;
         kernVD<float>(q, V, D);
         sv += (double)V;
-        sp += (double)(q * D);
+        sp = fmaf(q, D, sp);
         ++acc;
     }
 }
@@ -267,7 +270,9 @@ __global__ __launch_bounds__(kNrThreads) void densityKernel(DensArgs A)
                 if constexpr (kMixed)
                 {
                     imageFP64<PER>(dx, dy, dz, box);
-                    pairMixed(float(dx), float(dy), float(dz), hsr, hsrLo, s0, s1, acc);
+                    float sp = 0.0f;   // one pair per lane per tile
+                    pairMixed(float(dx), float(dy), float(dz), hsr, hsrLo, s0, sp, acc);
+                    s1 += (double)sp;
                 }
                 else
                 {
@@ -367,15 +372,17 @@ __global__ __launch_bounds__(kNrThreads) void densityKernel(DensArgs A)
                     {
                         const int4* ptr = &tq[w][first];
                         int4 nx = *ptr;
+                        float sp = 0.0f;
 #pragma unroll 4
                         for (int t = 0; t < steps; ++t)
                         {
                             const int4 q = nx;
                             ptr += stride;
                             nx = *ptr;
-                            pairMixed((float)(xq - q.x), (float)(yq - q.y), (float)(zq - q.z), hsr, hsrLo, s0, s1,
+                            pairMixed((float)(xq - q.x), (float)(yq - q.y), (float)(zq - q.z), hsr, hsrLo, s0, sp,
                                       acc);
                         }
+                        s1 += (double)sp;
                     }
                     else
                     {
