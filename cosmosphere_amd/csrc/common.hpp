@@ -89,16 +89,20 @@ __host__ __device__ inline void kernWdW(T q2, T q, T& w, T& dw)
 }
 
 // Branch-free pair terms of the mixed form (a wavefront whose q values straddle several
-// pieces would otherwise execute every piece), FP32, for 0 <= q <= R.  Both kernels are sums of
-// truncated powers (the reference's piecewise forms expanded, pinned by the CPU tests):
-//   quintic  W = a^5 - 6 b^5 + 15 c^5,  dW/dq = -5 (a^4 - 6 b^4 + 15 c^4),
-//            a = 3-q, b = (2-q)_+, c = (1-q)_+
-//   cubic    W = (a^3 - 4 b^3) / 4,     dW/dq = -3/4 (a^2 - 4 b^2),  a = 2-q, b = (1-q)_+
-// With V the bracket of W and D the bracket of dW/dq:
-//   W = kWw V   and   -q dW/dq - 3 W = kGp q D - kGw V,
-// so a pair loop accumulates V and q D and applies the constants once per particle.  The
-// coefficients are exact in FP32, and so is the evaluation at q = 0 (the self pair of every
-// particle): V(0) = 66 (quintic) or 4 (cubic), D(0) = 0.  a = R - q needs no clamp: q <= R.
+// pieces would otherwise execute every piece), FP32, for 0 <= q <= R, with q2 = q^2.
+// With V the bracket of W and D the bracket of dW/dq,
+//   quintic  W = V,       dW/dq = -5 D,    kWw = 1,    kGp = 5,    kGw = 3
+//   cubic    W = V / 4,   dW/dq = -3/4 D,  kWw = 1/4,  kGp = 3/4,  kGw = 3/4
+// so that W = kWw V and -q dW/dq - 3 W = kGp (q D) - kGw V: a pair loop accumulates V and q D
+// and applies the constants once per particle.  Two forms, selected without a branch:
+//   q >= 1   the truncated powers (the reference's outer pieces, include/kernel.hpp:35-78):
+//            quintic V = a^5 - 6 b^5, D = a^4 - 6 b^4 (a = 3-q, b = (2-q)_+);
+//            cubic   V = a^3,         D = a^2         (a = 2-q)
+//   q < 1    the inner piece expanded in q, free of the cancellation a^5 - 6 b^5 + 15 c^5 =
+//            243 - 192 + 15 that costs FP32 ~1.5e-6 of W(0) near q = 0:
+//            quintic V = 66 - 60 q^2 + 30 q^4 - 10 q^5,  q D = 24 q^2 - 24 q^4 + 10 q^5;
+//            cubic   V = 4 - 6 q^2 + 3 q^3,              q D = 4 q^2 - 3 q^3
+// All coefficients are exact in FP32, so V(0) (the self pair of every particle) is exact.
 #ifdef COSMO_KERNEL_QUINTIC
 constexpr double kWw = 1.0, kGp = 5.0, kGw = 3.0;
 #else
@@ -106,22 +110,27 @@ constexpr double kWw = 0.25, kGp = 0.75, kGw = 0.75;
 #endif
 
 template<class T>
-__host__ __device__ inline void kernVD(T q, T& V, T& D)
+__host__ __device__ inline void kernVD(T q, T q2, T& V, T& qD)
 {
+    const bool inner = q < T(1);
 #ifdef COSMO_KERNEL_QUINTIC
     const T a = T(3) - q;
     const T b = fmax(T(2) - q, T(0));
-    const T c = fmax(T(1) - q, T(0));
-    const T a2 = a * a, b2 = b * b, c2 = c * c;
-    const T a4 = a2 * a2, b4 = b2 * b2, c4 = c2 * c2;
-    V = fma(T(15), c4 * c, fma(T(-6), b4 * b, a4 * a));
-    D = fma(T(15), c4, fma(T(-6), b4, a4));
+    const T a2 = a * a, b2 = b * b;
+    const T a4 = a2 * a2, b4 = b2 * b2;
+    const T vOut = fma(T(-6), b4 * b, a4 * a);
+    const T dOut = fma(T(-6), b4, a4);
+    const T vIn  = fma(q2, fma(q2, fma(T(-10), q, T(30)), T(-60)), T(66));
+    const T qdIn = q2 * fma(q2, fma(T(10), q, T(-24)), T(24));
+    V  = inner ? vIn : vOut;
+    qD = inner ? qdIn : q * dOut;
 #else
     const T a = T(2) - q;
-    const T b = fmax(T(1) - q, T(0));
-    const T a2 = a * a, b2 = b * b;
-    V = fma(T(-4), b2 * b, a2 * a);
-    D = fma(T(-4), b2, a2);
+    const T a2 = a * a;
+    const T vIn  = fma(q2, fma(T(3), q, T(-6)), T(4));
+    const T qdIn = q2 * fma(T(-3), q, T(4));
+    V  = inner ? vIn : a2 * a;
+    qD = inner ? qdIn : q * a2;
 #endif
 }
 

@@ -186,7 +186,7 @@ __device__ inline void pairPlain(double dx, double dy, double dz, const DBox& bo
 }
 
 // Mixed pair term: FP32 separation (exact to one rounding), r^2, support test, q and kernel
-// brackets V, D (kernVD), when q < R.  V is summed in FP64 (it fixes rho(h), hence the root h);
+// brackets V, q D (kernVD), when q < R.  V is summed in FP64 (it fixes rho(h), hence the root h);
 // q D in the caller's FP32 partial (it sets d(rho)/d(h), which enters only the Newton step
 // length and the d(rho)/d(h) output), which the caller adds into FP64 every <= 64 pairs.
 // q^2 = r^2 (hsr + hsrLo), hsr + hsrLo = 1/h^2 in units of the separation's.
@@ -199,10 +199,10 @@ __device__ inline void pairMixed(float dx, float dy, float dz, float hsr, float 
     {
         q2 = fmaf(r2, hsrLo, q2);
         const float q = __builtin_amdgcn_sqrtf(q2);
-        float V, D;
-        kernVD<float>(q, V, D);
+        float V, qD;
+        kernVD<float>(q, q2, V, qD);
         sv += (double)V;
-        sp = fmaf(q, D, sp);
+        sp += qD;
         ++acc;
     }
 }

@@ -12,11 +12,11 @@ int main()
     {
         double w, dw;
         csph::kernWdW<double>(q * q, q, w, dw);
-        float V, D;
+        float V, qD;
         const float qf = (float)q;
-        csph::kernVD<float>(qf, V, D);
+        csph::kernVD<float>(qf, qf * qf, V, qD);
         const double wm = csph::kWw * (double)V;
-        const double gm = csph::kGp * (double)(qf * D) - csph::kGw * (double)V;
+        const double gm = csph::kGp * (double)qD - csph::kGw * (double)V;
         std::printf("%.17g %.17g %.17g %.17g\n", w, -q * dw - 3.0 * w, wm, gm);
     }
     return 0;

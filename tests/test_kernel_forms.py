@@ -2,9 +2,10 @@
 reference's piecewise form (kernWdW, include/kernel.hpp:35-78), both evaluated on the host from
 the library's own header, for the cubic and quintic builds.
 
-Pins: W = kWw V and -q dW/dq - 3W = kGp q D - kGw V to FP32 accuracy over [0, R]; exact at
-q = 0 (the self pair every particle's sum holds) and at the piece boundaries; no systematic
-bias (the mean error over the support is at the level of single roundings)."""
+Pins: W = kWw V and -q dW/dq - 3W = kGp (q D) - kGw V to a few FP32 roundings of W(0) over
+[0, R] (the inner piece has no cancellation); exact at q = 0 (the self pair every particle's sum
+holds) and at the piece boundaries; no systematic bias (the mean error over the support is at
+the level of single roundings)."""
 import os
 import shutil
 import subprocess
@@ -39,8 +40,9 @@ def test_forms_agree_to_fp32(forms):
     run, radius = forms
     q = np.linspace(0.0, radius, 200001).astype(np.float32).astype(np.float64)
     w, g, wm, gm = run(q)
-    assert np.max(np.abs(wm - w)) <= 4e-6 * w[0]
-    assert np.max(np.abs(gm - g)) <= 4e-6 * np.abs(g).max()
+    # a few FP32 roundings of W(0): the inner piece is evaluated without cancellation
+    assert np.max(np.abs(wm - w)) <= 3e-7 * w[0]
+    assert np.max(np.abs(gm - g)) <= 3e-7 * np.abs(g).max()
 
 
 def test_exact_at_self_pair_and_piece_boundaries(forms):
