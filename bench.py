@@ -107,6 +107,18 @@ def run_precision(lib, c, prec, steps, warmup):
     return res
 
 
+def measured_traffic(config, form):
+    """HBM bytes per launch of the density kernel for this workload and form, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/density_<config>_<form>_hbm_traffic.md),
+    or None when no such measurement exists."""
+    import re
+    path = os.path.join(ROOT, "profiles", "density_%s_%s_hbm_traffic.md" % (config.lower(), form))
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", open(path).read())
+    return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
+
+
 def rel_stats(a, b):
     e = np.abs(a - b) / np.abs(b)
     return {"max": float(e.max()), "p99": float(np.percentile(e, 99)), "p50": float(np.percentile(e, 50)),
@@ -290,6 +302,7 @@ def main():
     head = res.get("mixed") or res.get("fp64")
     head_name = "mixed" if "mixed" in res else "fp64"
     peak = FP32_PEAK if head_name == "mixed" else FP64_PEAK
+    traffic, traffic_src = measured_traffic(args.config, head_name)
     out = {
         "metric": METRIC,
         "value": head["h_solves_per_s"],
@@ -313,7 +326,8 @@ def main():
         "iterations": head["iterations"],
         "phase_ms": head["phase_ms"],
         "roofline": {"bound": "valu", "achieved": head["kernel_tflops"], "peak": peak, "unit": "TFLOP/s",
-                     "frac": head["kernel_tflops"] / peak, "traffic": None,
+                     "frac": head["kernel_tflops"] / peak, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "densityKernel (%s NR pair sums)" % head_name,
                      "kernel_avg_ms": head["kernel_avg_ms"],
                      "flops_convention": "9 per candidate + 21 per accepted pair"},

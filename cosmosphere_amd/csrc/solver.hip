@@ -238,7 +238,7 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         t.pairsCandidate += (double)c[kCtrCand];
         t.closurePairs += (double)c[kCtrClosPairs];
         t.closureCand += (double)c[kCtrClosCand];
-        t.kernLaunches += 1;
+        if (nrPass) t.kernLaunches += 1;   // density launches timed in nrkern
         nIterating = c[kCtrIterating];
         nItems = c[kCtrItems];
     }
