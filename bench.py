@@ -54,13 +54,15 @@ def load_config(name):
     raise SystemExit("unknown config " + name)
 
 
-def run_precision(lib, c, prec, steps, warmup):
+def run_precision(lib, c, prec, steps, warmup, opts=()):
     n = c["n"]
     lib.set_option("precision", prec)
     lib.set_option("max_iter", c.get("max_iter", 100))
     lib.set_option("closure", -1)
     lib.set_option("tree_only", 0)
     for k, v in DEFAULT_ITEMS.items():
+        lib.set_option(k, v)
+    for k, v in opts:
         lib.set_option(k, v)
 
     def one():
@@ -266,7 +268,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
+                    help="library option (cosmo_ext_set_option), e.g. loop=0 for the host-driven NR loop")
     args = ap.parse_args()
+    args.opts = [(kv.split("=")[0], float(kv.split("=")[1])) for kv in args.opt]
     if args.gpus != 1:
         print("bench.py measures one GPU: this solver runs a particle set (or an ensemble of sets) on one MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -293,7 +298,7 @@ def main():
         if args.only not in ("both", name):
             continue
         log("precision %s" % name)
-        res[name] = run_precision(lib, c, prec, args.steps, args.warmup)
+        res[name] = run_precision(lib, c, prec, args.steps, args.warmup, args.opts)
         r = res[name]
         log("  %s: %.1f ms/step, %.3e h-solves/s, %.3e pairs/s (kernel), kernel %.2f TFLOP/s, iters %.1f"
             % (name, r["ms_per_step"], r["h_solves_per_s"], r["pairs_per_s_kernel"], r["kernel_tflops"],

@@ -328,6 +328,12 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "tail_leaves") s.tailLeaves = (int)value;
     else if (nm == "closure") s.closureOn = (int)value;
     else if (nm == "tree_only") s.treeOnly = (int)value;
+    else if (nm == "loop") s.loopMode = (int)value;
+    else if (nm == "graph") s.useGraph = (int)value;
+    else if (nm == "graph_passes") s.graphPasses = std::max(1, (int)value);
+    else if (nm == "lpt") s.lpt = (int)value;
+    else if (nm == "loop_blocks_scale") s.loopBlockScale = std::max(1, (int)value);
+    else if (nm == "loop_list_cap") s.loopListCap = (long long)value;
     else
     {
         std::fprintf(stderr, "FATAL: cosmo_ext_set_option: unknown option '%s'\n", name);
@@ -374,6 +380,16 @@ extern "C" double cosmo_ext_get_stat(const char* name)
     if (nm == "list_leaves") return s.listLeaves;
     if (nm == "precision") return s.precision;
     if (nm == "max_iter") return s.maxIter;
+    if (nm == "graph_replays") return t.graphReplays;
+    if (nm == "graph_captures") return t.graphCaptures;
+    if (nm == "loop") return s.loopMode;
+    if (nm == "graph") return s.useGraph;
+    if (nm == "graph_passes") return s.graphPasses;
+    if (nm == "lpt") return s.lpt;
+    if (nm == "mapping") return s.mapping;
+    if (nm == "sparse") return s.sparse;
+    if (nm == "tail_leaves") return s.tailLeaves;
+    if (nm == "loop_blocks_scale") return s.loopBlockScale;
     return -1.0;
 }
 

@@ -10,6 +10,36 @@
 namespace csph
 {
 
+// Device-resident state of the Newton-Raphson loop when its passes run without the host
+// (loop.hip): every kernel of a pass reads its counts and its go/no-go from here instead of
+// from its launch size.  Written only by the one-thread bookkeeping kernels of a pass
+// (passBegin / listCheck / itemCount / passEnd) and by the work queue's atomics.
+struct PassCtl
+{
+    uint32_t nAL;           // active leaves: positions [0, nAL) of the leaf list
+    uint32_t nItems;        // work items of the current pass
+    uint32_t iters;         // NR passes completed
+    uint32_t passes;        // passes completed (NR and closure-only): next log record
+    uint32_t halt;          // the loop is over: no active leaf, or the NR pass cap reached
+    uint32_t overflow;      // the current pass needs more list entries than the list buffer holds
+    uint32_t skip;          // the current pass does nothing (halt or overflow)
+    uint32_t perParticle;   // the current pass uses per-particle items for sparse leaves
+    uint32_t isNR;          // the current pass holds NR evaluations (else closures only)
+    uint32_t queueHead;     // next position of the work queue
+    uint32_t qBucket[32];   // cost-order buckets: counts, then running bases
+    unsigned long long nIterating;      // particles in state 1 at the start of the pass
+    unsigned long long listNeed;        // list entries of the current pass
+    unsigned long long tBegin;          // wall clock at the start of the pass
+    unsigned long long tStart, tEnd;    // density kernel span (wall clock) of the current pass
+};
+
+// One record per pass of a device-driven loop (u64 fields, wall-clock ticks for times).
+struct PassLogDev
+{
+    unsigned long long nAL, nItems, nIterating, cand, pairs, closCand, closPairs;
+    unsigned long long tBegin, tKStart, tKEnd, tEnd, perParticle, listTotal, isNR, pad0, pad1;
+};
+
 struct DensArgs
 {
     const int*      leaves;     // active leaf ids (nullptr: leaf a is leaf a)
@@ -35,6 +65,8 @@ struct DensArgs
     const double*   pmassSet;   // ensemble: particle mass of each set (nullptr: pmass)
     int             closure;    // converged particles pass through state 2
     unsigned long long* counters;
+    PassCtl*        ctl;        // device-driven pass: persistent waves pull items from a queue
+    const int*      qOrder;     // ... in this order (cost-descending), nullptr: item order
 };
 
 // Device work counters (u64, summed over the counter slots).
@@ -90,18 +122,25 @@ struct ForceArgs
 // 1: NR on every particle of an active leaf (WARP_PER_LEAF); 2: closure of every particle,
 // rho and d(rho)/d(h) at the current h, no update.  a.nItems work items.
 void launchDensity(const DensArgs& a, int precision, int mode, bool periodic, hipStream_t st);
+// Device-driven pass: both item forms are launched as persistent grids of `blocks` blocks; the
+// one ctl->perParticle does not select exits at once.
+void launchDensityPersistent(const DensArgs& a, int precision, bool periodic, int blocks, hipStream_t st);
+int  densityPersistentBlocks(int precision, bool periodic);
 void launchGradients(const GradArgs& a, bool periodic, hipStream_t st);
 void launchDeadGradients(int first, int n, double* divv, double* xi, double* ddivvdt, hipStream_t st);
 // Per active leaf: flag (has a particle in state != 0) and its work-item count (1 whole-leaf
 // item, or one item per remaining particle when at most `sparse` remain); counters 2-4.
+// With ctl (device-driven pass), nLeaves is the capacity and ctl->nAL the count.
 void launchLeafFlags(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* state, int sparse,
-                     uint32_t* flag, uint32_t* itemCount, unsigned long long* counters, hipStream_t st);
+                     uint32_t* flag, uint32_t* itemCount, unsigned long long* counters, hipStream_t st,
+                     const PassCtl* ctl = nullptr);
 // Compaction: out[pos[a]] = leaves[a], outCount[pos[a]] = count[a] for flagged a.
 void launchLeafScatter(const int* leaves, const uint32_t* count, int nLeaves, const uint32_t* flag,
-                       const uint32_t* pos, int* out, uint32_t* outCount, hipStream_t st);
+                       const uint32_t* pos, int* out, uint32_t* outCount, hipStream_t st,
+                       const PassCtl* ctl = nullptr);
 // Work items of the active leaves from their counts' exclusive scan `off`.
 void launchItemFill(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* state, int sparse,
-                    const uint32_t* off, int* itemPos, int* itemPart, hipStream_t st);
+                    const uint32_t* off, int* itemPos, int* itemPart, hipStream_t st, const PassCtl* ctl = nullptr);
 void launchInitActive(uint8_t* active, int nAlive, int n, double* rho, double* gradh, hipStream_t st);
 void launchIota(int* v, int n, hipStream_t st);
 void launchMaxH(const double* h, int n, unsigned long long* out, hipStream_t st);

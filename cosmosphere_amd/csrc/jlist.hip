@@ -27,9 +27,12 @@
  * such pairs, in the same depth-first order: the accepted pairs and their order are
  * unchanged.
  */
+#include "density.hpp"
 #include "tile.hpp"
 #include "tree.hpp"
 
+#include <algorithm>
+#include <cstring>
 #include <vector>
 
 namespace csph
@@ -41,28 +44,31 @@ namespace csph
 __global__ __launch_bounds__(256) void hmaxLeafKernel(const double* __restrict__ h, const uint32_t* __restrict__ layout,
                                                       const int* __restrict__ leaves, int nLeaves,
                                                       double* __restrict__ hmaxLeaf, const uint8_t* __restrict__ state,
-                                                      double* __restrict__ hRem)
+                                                      double* __restrict__ hRem, const PassCtl* __restrict__ ctl)
 {
-    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (a >= nLeaves) return;
-    const int l = leaves ? leaves[a] : a;
-    const uint32_t p0 = layout[l], p1 = layout[l + 1];
-    double m = 0.0, r = 0.0;
-    for (uint32_t p = p0 + laneId(); p < p1; p += kWave)
+    if (ctl && ctl->skip) return;
+    const int n = ctl ? (int)ctl->nAL : nLeaves;
+    for (int a = blockIdx.x * 4 + (threadIdx.x >> 6); a < n; a += gridDim.x * 4)
     {
-        const double v = h[p];
-        if (isfinite(v))
+        const int l = leaves ? leaves[a] : a;
+        const uint32_t p0 = layout[l], p1 = layout[l + 1];
+        double m = 0.0, r = 0.0;
+        for (uint32_t p = p0 + laneId(); p < p1; p += kWave)
         {
-            m = fmax(m, v);
-            if (state && state[p] != 0) r = fmax(r, v);
+            const double v = h[p];
+            if (isfinite(v))
+            {
+                m = fmax(m, v);
+                if (state && state[p] != 0) r = fmax(r, v);
+            }
         }
-    }
-    m = waveReduceMax(m);
-    if (laneId() == 0) hmaxLeaf[l] = m;
-    if (state)
-    {
-        r = waveReduceMax(r);
-        if (laneId() == 0) hRem[l] = r;
+        m = waveReduceMax(m);
+        if (laneId() == 0) hmaxLeaf[l] = m;
+        if (state)
+        {
+            r = waveReduceMax(r);
+            if (laneId() == 0) hRem[l] = r;
+        }
     }
 }
 
@@ -135,6 +141,7 @@ struct ListArgs
     uint32_t*       jCount;
     int*            ovf;
     int             nNodes;   // walk cap: a tree walk visits each node at most once
+    const PassCtl*  ctl;      // device-driven pass: ctl->nAL leaves, nothing when ctl->skip
 };
 
 template<bool SYM, bool COUNT, bool PER>
@@ -143,10 +150,12 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
 #pragma clang fp contract(off)
     __shared__ int stk[kGroups][kStack];
     const int g = threadIdx.x >> 3, o = threadIdx.x & 7;
-    const int a = blockIdx.x * kGroups + g;
-    if (a >= A.nLeaves) return;
+    if (A.ctl && A.ctl->skip) return;
+    const int nL = A.ctl ? (int)A.ctl->nAL : A.nLeaves;
     const int shift = (laneId() >> 3) << 3;
     const uint32_t below = (1u << o) - 1u;
+    for (int a = blockIdx.x * kGroups + g; a < nL; a += gridDim.x * kGroups)
+    {
 
     const int iLeaf = A.leaves ? A.leaves[a] : a;
     const int in = A.leafNode[iLeaf];
@@ -192,7 +201,7 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
     if (!inRange(root))
     {
         if (COUNT && o == 0) A.jCount[a] = 0;
-        return;
+        continue;
     }
     const int rc = A.nodeChild[root];
     if (rc < 0)
@@ -202,7 +211,7 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
             if (COUNT) A.jCount[a] = 1;
             else       A.jList[base] = ~rc;
         }
-        return;
+        continue;
     }
 
     const uint32_t limit = COUNT ? 0u : A.jOffset[a + 1];
@@ -245,18 +254,20 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
         waveLdsSync();
     }
     if (COUNT && o == 0) A.jCount[a] = count;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------------------
-void hmaxLeaves(State& s, const int* leaves, int nLeaves, const uint8_t* state)
+void hmaxLeaves(State& s, const int* leaves, int nLeaves, const uint8_t* state, const PassCtl* ctl)
 {
     if (nLeaves <= 0) return;
     double* hm = s.hmaxLeaf.get<double>(s.nLeaves + 1);
     double* hr = state ? s.hRemLeaf.get<double>(s.nLeaves + 1) : nullptr;
-    hmaxLeafKernel<<<iceil(nLeaves, 4), 256, 0, s.stream>>>(s.h.as<double>(), s.layout.as<uint32_t>(), leaves, nLeaves,
-                                                             hm, state, hr);
+    const int grid = ctl ? std::min(iceil(nLeaves, 4), 4096) : iceil(nLeaves, 4);
+    hmaxLeafKernel<<<grid, 256, 0, s.stream>>>(s.h.as<double>(), s.layout.as<uint32_t>(), leaves, nLeaves, hm, state,
+                                               hr, ctl);
     LAUNCH_CHECK();
 }
 
@@ -278,18 +289,16 @@ void hmaxUpsweep(State& s)
 template<bool SYM, bool COUNT>
 static void launchList(const ListArgs& a, bool periodic, hipStream_t st)
 {
-    const int grid = iceil(a.nLeaves, kGroups);
+    const int grid = a.ctl ? std::min(iceil(a.nLeaves, kGroups), 8192) : iceil(a.nLeaves, kGroups);
     if (periodic) jlistKernel<SYM, COUNT, true><<<grid, kListThreads, 0, st>>>(a);
     else          jlistKernel<SYM, COUNT, false><<<grid, kListThreads, 0, st>>>(a);
     LAUNCH_CHECK();
 }
 
-long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric, bool tight)
+static ListArgs listArgs(State& s, const int* leaves, int nLeaves, bool symmetric, bool tight)
 {
-    hipStream_t st = s.stream;
-    uint32_t* jc = s.jCount.get<uint32_t>(nLeaves + 1);
-    uint32_t* jo = s.jOffset.get<uint32_t>(nLeaves + 2);
-    ListArgs a{};
+    ListArgs a;
+    std::memset(&a, 0, sizeof(a));
     a.leaves = leaves;
     a.nLeaves = nLeaves;
     a.leafNode = s.leafNode.as<int>();
@@ -302,9 +311,32 @@ long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric, b
     a.hRemLeaf = (tight && !symmetric) ? s.hRemLeaf.as<double>() : nullptr;
     a.nodeHmax = symmetric ? s.nodeHmax.as<double>() : nullptr;
     a.boxes = s.box.as<DBox>();
-    a.jCount = jc;
     a.ovf = s.ovf.get<int>(4);
     a.nNodes = s.nNodes;
+    return a;
+}
+
+void launchGatherListsDevice(State& s, const int* leaves, int nCap, uint32_t* jCount, const uint32_t* jOffset,
+                             int* jList, const PassCtl* ctl)
+{
+    if (nCap <= 0) return;
+    ListArgs a = listArgs(s, leaves, nCap, false, true);
+    a.ctl = ctl;
+    a.jCount = jCount;
+    a.jOffset = jOffset;
+    a.jList = jList;
+    const bool per = s.hBox.size() && s.hBox[0].periodic;
+    if (jList) launchList<false, false>(a, per, s.stream);
+    else       launchList<false, true>(a, per, s.stream);
+}
+
+long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric, bool tight)
+{
+    hipStream_t st = s.stream;
+    uint32_t* jc = s.jCount.get<uint32_t>(nLeaves + 1);
+    uint32_t* jo = s.jOffset.get<uint32_t>(nLeaves + 2);
+    ListArgs a = listArgs(s, leaves, nLeaves, symmetric, tight);
+    a.jCount = jc;
     const bool per = s.hBox.size() && s.hBox[0].periodic;
     if (nLeaves > 0)
     {

@@ -45,6 +45,7 @@
 #include "prims.hpp"
 #include "tile.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace csph
@@ -225,8 +226,11 @@ __device__ inline void particleSums(double s0, double s1, double& wsum, double& 
 
 // PP: the launch holds per-particle items (a separate instance keeps the register budget of
 // the whole-leaf path low when it does not).
-template<class Real, bool PER, int MODE, bool PP>
-__global__ __launch_bounds__(kNrThreads) void densityKernel(DensArgs A)
+// QUEUE: the device-driven pass's persistent grid (A.ctl), a separate instance so the
+// register budget of the host-driven kernels is unchanged; held to 80 VGPRs (6 waves per SIMD).
+template<class Real, bool PER, int MODE, bool PP, bool QUEUE = false>
+__global__ __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(QUEUE ? 6 : 1)))
+void densityKernel(DensArgs A)
 {
     constexpr bool kMixed = sizeof(Real) == 4;
     __shared__ int segStart[kNrWaves][kWave];
@@ -240,11 +244,9 @@ __global__ __launch_bounds__(kNrThreads) void densityKernel(DensArgs A)
     __shared__ double tz[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
 
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = laneId();
-    const int item = blockIdx.x * kNrWaves + w;
     unsigned long long ctr[4] = {0, 0, 0, 0};   // NR pairs, NR candidates, closure pairs, closure candidates
 
-    if (item < A.nItems)
-    {
+    auto run = [&](const int item, unsigned long long (&cc)[4]) {
         const int a = PP ? A.itemPos[item] : item;
         const int part = PP ? __builtin_amdgcn_readfirstlane(A.itemPart[item]) : -1;
         const int iLeaf = A.leaves ? A.leaves[a] : a;
@@ -284,8 +286,8 @@ __global__ __launch_bounds__(kNrThreads) void densityKernel(DensArgs A)
             s0 = waveReduceSum(s0);
             s1 = waveReduceSum(s1);
             const int slot = isClosureEval<MODE>(st) ? 2 : 0;
-            ctr[slot] = (unsigned long long)acc;
-            ctr[slot + 1] = cand;
+            cc[slot] += (unsigned long long)acc;
+            cc[slot + 1] += cand;
             double wsum, gsum;
             particleSums<kMixed>(s0, s1, wsum, gsum);
             if (lane == 0) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
@@ -410,8 +412,8 @@ __global__ __launch_bounds__(kNrThreads) void densityKernel(DensArgs A)
                 if (has)
                 {
                     const int slot = isClosureEval<MODE>(st) ? 2 : 0;
-                    ctr[slot] += (unsigned long long)acc;
-                    ctr[slot + 1] += cand;
+                    cc[slot] += (unsigned long long)acc;
+                    cc[slot + 1] += cand;
                 }
                 // group sums, a tree over lanes sub .. g-1 of each group (lane sub = 0 ends with it)
                 for (int off = 1; off < g; off <<= 1)
@@ -424,8 +426,32 @@ __global__ __launch_bounds__(kNrThreads) void densityKernel(DensArgs A)
                 if (has && sub == 0) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
             }
         }
+    };
+
+    if constexpr (QUEUE)
+    {
+        // device-driven pass: a persistent grid; each wavefront pulls the next item of the
+        // pass's queue (cost-descending when qOrder is given) until the queue is empty
+        PassCtl* const C = A.ctl;
+        if (C->skip || C->perParticle != (PP ? 1u : 0u)) return;   // the whole block
+        if (threadIdx.x == 0) atomicMin(&C->tStart, (unsigned long long)wall_clock64());
+        const int nItems = (int)C->nItems;
+        for (;;)
+        {
+            int k = 0;
+            if (lane == 0) k = (int)atomicAdd(&C->queueHead, 1u);
+            k = __builtin_amdgcn_readfirstlane(k);
+            if (k >= nItems) break;
+            run(A.qOrder ? A.qOrder[k] : k, ctr);
+        }
+    }
+    else
+    {
+        const int item = blockIdx.x * kNrWaves + w;
+        if (item < A.nItems) run(item, ctr);
     }
     blockCounters(A.counters, ctr);
+    if (QUEUE && threadIdx.x == 0) atomicMax(&A.ctl->tEnd, (unsigned long long)wall_clock64());
 }
 
 // ---------------------------------------------------------------------------------------
@@ -442,42 +468,51 @@ __global__ __launch_bounds__(256) void leafFlagKernel(const int* __restrict__ le
                                                       const uint32_t* __restrict__ layout,
                                                       const uint8_t* __restrict__ state, int sparse,
                                                       uint32_t* __restrict__ flag, uint32_t* __restrict__ count,
-                                                      unsigned long long* __restrict__ counters)
+                                                      unsigned long long* __restrict__ counters,
+                                                      const PassCtl* __restrict__ ctl)
 {
-    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (a >= nLeaves) return;
-    const int l = leaves[a];
-    const uint32_t p0 = layout[l], p1 = layout[l + 1];
-    uint32_t it = 0, rem = 0;
-    for (uint32_t p = p0 + laneId(); p < p1; p += kWave)
+    if (ctl && ctl->skip) return;
+    const int n = ctl ? (int)ctl->nAL : nLeaves;
+    unsigned long long* slot = counters + (blockIdx.x % kCounterSlots) * kCounterStride;
+    for (int a = blockIdx.x * 4 + (threadIdx.x >> 6); a < n; a += gridDim.x * 4)
     {
-        const uint8_t s = state[p];
-        it += (s == 1) ? 1u : 0u;
-        rem += s ? 1u : 0u;
-    }
-    it = waveReduceSum(it);
-    rem = waveReduceSum(rem);
-    if (laneId() == 0)
-    {
-        const uint32_t items = itemsOf(rem, sparse);
-        flag[a] = rem ? 1u : 0u;
-        count[a] = items;
-        unsigned long long* slot = counters + (blockIdx.x % kCounterSlots) * kCounterStride;
-        if (it) atomicAdd(slot + kCtrIterating, (unsigned long long)it);
-        if (rem) atomicAdd(slot + kCtrRemaining, (unsigned long long)rem);
-        if (items) atomicAdd(slot + kCtrItems, (unsigned long long)items);
+        const int l = leaves[a];
+        const uint32_t p0 = layout[l], p1 = layout[l + 1];
+        uint32_t it = 0, rem = 0;
+        for (uint32_t p = p0 + laneId(); p < p1; p += kWave)
+        {
+            const uint8_t s = state[p];
+            it += (s == 1) ? 1u : 0u;
+            rem += s ? 1u : 0u;
+        }
+        it = waveReduceSum(it);
+        rem = waveReduceSum(rem);
+        if (laneId() == 0)
+        {
+            const uint32_t items = itemsOf(rem, sparse);
+            flag[a] = rem ? 1u : 0u;
+            count[a] = items;
+            if (it) atomicAdd(slot + kCtrIterating, (unsigned long long)it);
+            if (rem) atomicAdd(slot + kCtrRemaining, (unsigned long long)rem);
+            if (items) atomicAdd(slot + kCtrItems, (unsigned long long)items);
+        }
     }
 }
 
 __global__ void leafScatterKernel(const int* __restrict__ leaves, const uint32_t* __restrict__ count, int nLeaves,
                                   const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
-                                  int* __restrict__ out, uint32_t* __restrict__ outCount)
+                                  int* __restrict__ out, uint32_t* __restrict__ outCount,
+                                  const PassCtl* __restrict__ ctl)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < nLeaves && flag[a])
+    if (ctl && ctl->skip) return;
+    const int n = ctl ? (int)ctl->nAL : nLeaves;
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x)
     {
-        out[pos[a]] = leaves[a];
-        outCount[pos[a]] = count[a];
+        if (flag[a])
+        {
+            out[pos[a]] = leaves[a];
+            outCount[pos[a]] = count[a];
+        }
     }
 }
 
@@ -487,34 +522,37 @@ __global__ __launch_bounds__(256) void itemFillKernel(const int* __restrict__ le
                                                       const uint32_t* __restrict__ layout,
                                                       const uint8_t* __restrict__ state, int sparse,
                                                       const uint32_t* __restrict__ off, int* __restrict__ itemPos,
-                                                      int* __restrict__ itemPart)
+                                                      int* __restrict__ itemPart, const PassCtl* __restrict__ ctl)
 {
-    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (a >= nLeaves) return;
-    const int l = leaves[a];
-    const uint32_t p0 = layout[l], p1 = layout[l + 1];
-    uint32_t rem = 0;
-    for (uint32_t p = p0 + laneId(); p < p1; p += kWave) rem += state[p] ? 1u : 0u;
-    rem = waveReduceSum(rem);
-    if (rem == 0) return;
-    uint32_t o = off[a];
-    if (rem > (uint32_t)sparse)
+    if (ctl && (ctl->skip || !ctl->perParticle)) return;
+    const int n = ctl ? (int)ctl->nAL : nLeaves;
+    for (int a = blockIdx.x * 4 + (threadIdx.x >> 6); a < n; a += gridDim.x * 4)
     {
-        if (laneId() == 0) { itemPos[o] = a; itemPart[o] = -1; }
-        return;
-    }
-    for (uint32_t q0 = p0; q0 < p1; q0 += kWave)
-    {
-        const uint32_t p = q0 + laneId();
-        const bool live = p < p1 && state[p] != 0;
-        const uint64_t m = ballot64(live);
-        if (live)
+        const int l = leaves[a];
+        const uint32_t p0 = layout[l], p1 = layout[l + 1];
+        uint32_t rem = 0;
+        for (uint32_t p = p0 + laneId(); p < p1; p += kWave) rem += state[p] ? 1u : 0u;
+        rem = waveReduceSum(rem);
+        if (rem == 0) continue;
+        uint32_t o = off[a];
+        if (rem > (uint32_t)sparse)
         {
-            const uint32_t k = o + (uint32_t)__popcll(m & lanemaskLt());
-            itemPos[k] = a;
-            itemPart[k] = (int)p;
+            if (laneId() == 0) { itemPos[o] = a; itemPart[o] = -1; }
+            continue;
         }
-        o += (uint32_t)__popcll(m);
+        for (uint32_t q0 = p0; q0 < p1; q0 += kWave)
+        {
+            const uint32_t p = q0 + laneId();
+            const bool live = p < p1 && state[p] != 0;
+            const uint64_t m = ballot64(live);
+            if (live)
+            {
+                const uint32_t k = o + (uint32_t)__popcll(m & lanemaskLt());
+                itemPos[k] = a;
+                itemPart[k] = (int)p;
+            }
+            o += (uint32_t)__popcll(m);
+        }
     }
 }
 
@@ -554,31 +592,78 @@ void launchDensity(const DensArgs& a, int precision, int mode, bool periodic, hi
     LAUNCH_CHECK();
 }
 
+void launchDensityPersistent(const DensArgs& a, int precision, bool periodic, int blocks, hipStream_t st)
+{
+#define CSPH_DENS_P(R, P)                                                   \
+    do {                                                                    \
+        densityKernel<R, P, 0, false, true><<<blocks, kNrThreads, 0, st>>>(a); \
+        densityKernel<R, P, 0, true, true><<<blocks, kNrThreads, 0, st>>>(a);  \
+    } while (0)
+    if (precision == kMixed)
+    {
+        if (periodic) CSPH_DENS_P(float, true);
+        else          CSPH_DENS_P(float, false);
+    }
+    else
+    {
+        if (periodic) CSPH_DENS_P(double, true);
+        else          CSPH_DENS_P(double, false);
+    }
+#undef CSPH_DENS_P
+    LAUNCH_CHECK();
+}
+
+// Resident blocks of the whole-leaf instance over the chip (the persistent grid).
+int densityPersistentBlocks(int precision, bool periodic)
+{
+    int dev = 0, cus = 0, per = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const void* f = precision == kMixed
+                        ? (periodic ? (const void*)densityKernel<float, true, 0, false, true>
+                                    : (const void*)densityKernel<float, false, 0, false, true>)
+                        : (periodic ? (const void*)densityKernel<double, true, 0, false, true>
+                                    : (const void*)densityKernel<double, false, 0, false, true>);
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, kNrThreads, 0));
+    return std::max(cus, 1) * std::max(per, 1);
+}
+
 int counterSlots() { return kCounterSlots; }
 int counterStride() { return kCounterStride; }
 
+// With ctl the grids are capped (grid-stride loops over ctl->nAL), so a device-driven pass
+// launches a fixed shape whatever the count.
+static int strideGrid(int units, int perBlock, const PassCtl* ctl)
+{
+    const int g = iceil(units, perBlock);
+    return ctl ? std::min(g, 4096) : g;
+}
+
 void launchLeafFlags(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* state, int sparse,
-                     uint32_t* flag, uint32_t* itemCount, unsigned long long* counters, hipStream_t st)
+                     uint32_t* flag, uint32_t* itemCount, unsigned long long* counters, hipStream_t st,
+                     const PassCtl* ctl)
 {
     if (nLeaves <= 0) return;
-    leafFlagKernel<<<iceil(nLeaves, 4), 256, 0, st>>>(leaves, nLeaves, layout, state, sparse, flag, itemCount,
-                                                      counters);
+    leafFlagKernel<<<strideGrid(nLeaves, 4, ctl), 256, 0, st>>>(leaves, nLeaves, layout, state, sparse, flag,
+                                                                itemCount, counters, ctl);
     LAUNCH_CHECK();
 }
 
 void launchLeafScatter(const int* leaves, const uint32_t* count, int nLeaves, const uint32_t* flag,
-                       const uint32_t* pos, int* out, uint32_t* outCount, hipStream_t st)
+                       const uint32_t* pos, int* out, uint32_t* outCount, hipStream_t st, const PassCtl* ctl)
 {
     if (nLeaves <= 0) return;
-    leafScatterKernel<<<iceil(nLeaves, 256), 256, 0, st>>>(leaves, count, nLeaves, flag, pos, out, outCount);
+    leafScatterKernel<<<strideGrid(nLeaves, 256, ctl), 256, 0, st>>>(leaves, count, nLeaves, flag, pos, out,
+                                                                     outCount, ctl);
     LAUNCH_CHECK();
 }
 
 void launchItemFill(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* state, int sparse,
-                    const uint32_t* off, int* itemPos, int* itemPart, hipStream_t st)
+                    const uint32_t* off, int* itemPos, int* itemPart, hipStream_t st, const PassCtl* ctl)
 {
     if (nLeaves <= 0) return;
-    itemFillKernel<<<iceil(nLeaves, 4), 256, 0, st>>>(leaves, nLeaves, layout, state, sparse, off, itemPos, itemPart);
+    itemFillKernel<<<strideGrid(nLeaves, 4, ctl), 256, 0, st>>>(leaves, nLeaves, layout, state, sparse, off,
+                                                                itemPos, itemPart, ctl);
     LAUNCH_CHECK();
 }
 

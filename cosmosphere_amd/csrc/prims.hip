@@ -14,6 +14,7 @@
  *                carries ranks across rounds -- so the order within a digit is the input
  *                order, which is what makes the sort stable.
  */
+#include "density.hpp"
 #include "prims.hpp"
 
 #include <algorithm>
@@ -28,12 +29,20 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems   = 8;
 constexpr int kScanTile    = kScanThreads * kScanItems;
 
+// Count of a device-driven scan: entries from nEff on read as zero; tiles past nEff are
+// not written (only out[0..nEff] is defined).
+__device__ inline int scanCount(const PassCtl* ctl, int n) { return ctl ? min(n, (int)ctl->nAL) : n; }
+
 template<class T>
 __global__ __launch_bounds__(kScanThreads) void scanTileKernel(const T* __restrict__ in, T* __restrict__ out,
                                                                T* __restrict__ blockSums, int n,
-                                                               unsigned long long* __restrict__ total64)
+                                                               unsigned long long* __restrict__ total64,
+                                                               const PassCtl* __restrict__ ctl)
 {
     __shared__ T waveTot[kScanThreads / kWave];
+    if (ctl && ctl->skip) return;
+    const int nEff = scanCount(ctl, n);
+    if (ctl && (int)blockIdx.x * kScanTile > nEff) return;   // the whole block
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     T v[kScanItems];
     T sum = 0;
@@ -41,7 +50,7 @@ __global__ __launch_bounds__(kScanThreads) void scanTileKernel(const T* __restri
     for (int k = 0; k < kScanItems; ++k)
     {
         const int i = base + k;
-        v[k] = (i < n) ? in[i] : T(0);
+        v[k] = (i < nEff) ? in[i] : T(0);
         sum += v[k];
     }
     // block-wide exclusive scan of the per-thread sums
@@ -72,16 +81,19 @@ __global__ __launch_bounds__(kScanThreads) void scanTileKernel(const T* __restri
 }
 
 template<class T>
-__global__ void scanAddKernel(T* __restrict__ out, const T* __restrict__ blockOffs, int n)
+__global__ void scanAddKernel(T* __restrict__ out, const T* __restrict__ blockOffs, int n,
+                              const PassCtl* __restrict__ ctl)
 {
+    if (ctl && ctl->skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += blockOffs[i / kScanTile];
+    if (i < n && (!ctl || i <= scanCount(ctl, n))) out[i] += blockOffs[i / kScanTile];
 }
 
 template<class T>
-__global__ void scanFinishKernel(const T* __restrict__ in, T* __restrict__ out, int n)
+__global__ void scanFinishKernel(const T* __restrict__ in, T* __restrict__ out, int n, const PassCtl* __restrict__ ctl)
 {
-    // out[n] = out[n-1] + in[n-1]
+    // out[n] = out[n-1] + in[n-1] (device-driven: only when every entry counts)
+    if (ctl && (ctl->skip || scanCount(ctl, n) < n)) return;
     if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = (n > 0) ? out[n - 1] + in[n - 1] : T(0);
 }
 
@@ -96,13 +108,17 @@ size_t scanScratchBytes(int n)
     return bytes + 1024;
 }
 
+// The block sums of a device-driven scan are complete up to the tile holding nEff; the
+// levels above scan them over their full length (the sums of tiles past nEff are stale and
+// only reach offsets no kernel reads).
 template<class T>
-static void scanRec(const T* in, T* out, int n, char* scratch, unsigned long long* total64, hipStream_t st)
+static void scanRec(const T* in, T* out, int n, char* scratch, unsigned long long* total64, hipStream_t st,
+                    const PassCtl* ctl)
 {
     const int nb = iceil(n, kScanTile);
     if (nb <= 1)
     {
-        scanTileKernel<T><<<1, kScanThreads, 0, st>>>(in, out, nullptr, n, total64);
+        scanTileKernel<T><<<1, kScanThreads, 0, st>>>(in, out, nullptr, n, total64, ctl);
         LAUNCH_CHECK();
         return;
     }
@@ -110,24 +126,26 @@ static void scanRec(const T* in, T* out, int n, char* scratch, unsigned long lon
     T* offs = sums + nb + 1;
     char* next = reinterpret_cast<char*>(offs + nb + 1);
     next = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(next) + 255) & ~uintptr_t(255));
-    scanTileKernel<T><<<nb, kScanThreads, 0, st>>>(in, out, sums, n, total64);
+    scanTileKernel<T><<<nb, kScanThreads, 0, st>>>(in, out, sums, n, total64, ctl);
     LAUNCH_CHECK();
-    scanRec<T>(sums, offs, nb, next, nullptr, st);
-    scanAddKernel<T><<<iceil(n, 256), 256, 0, st>>>(out, offs, n);
+    scanRec<T>(sums, offs, nb, next, nullptr, st, nullptr);
+    scanAddKernel<T><<<iceil(n, 256), 256, 0, st>>>(out, offs, n, ctl);
     LAUNCH_CHECK();
 }
 
 template<class T>
-void scanExclusive(const T* in, T* out, int n, DevBuf& scratch, unsigned long long* total64, hipStream_t st)
+void scanExclusive(const T* in, T* out, int n, DevBuf& scratch, unsigned long long* total64, hipStream_t st,
+                   const PassCtl* ctl)
 {
     char* sc = scratch.get<char>(scanScratchBytes(n));
-    if (n > 0) scanRec<T>(in, out, n, sc, total64, st);
-    scanFinishKernel<T><<<1, 64, 0, st>>>(in, out, n);
+    if (n > 0) scanRec<T>(in, out, n, sc, total64, st, ctl);
+    scanFinishKernel<T><<<1, 64, 0, st>>>(in, out, n, ctl);
     LAUNCH_CHECK();
 }
 
-template void scanExclusive<uint32_t>(const uint32_t*, uint32_t*, int, DevBuf&, unsigned long long*, hipStream_t);
-template void scanExclusive<int>(const int*, int*, int, DevBuf&, unsigned long long*, hipStream_t);
+template void scanExclusive<uint32_t>(const uint32_t*, uint32_t*, int, DevBuf&, unsigned long long*, hipStream_t,
+                                      const PassCtl*);
+template void scanExclusive<int>(const int*, int*, int, DevBuf&, unsigned long long*, hipStream_t, const PassCtl*);
 
 // ---------------------------------------------------------------------------------------
 // Key spread: OR over i of (key_i XOR key_0), so a digit no key differs in is skipped.

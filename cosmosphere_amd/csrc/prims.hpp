@@ -8,6 +8,8 @@
 namespace csph
 {
 
+struct PassCtl;
+
 struct SortScratch
 {
     DevBuf hist, offs, scan, counter;
@@ -18,8 +20,11 @@ size_t scanScratchBytes(int n);
 
 // out[0..n] = exclusive prefix sums of in[0..n), out[n] = total.  If total64 is given,
 // the total is also added to it as a 64-bit sum (overflow check).
+// With ctl (device-driven pass) n is the capacity: entries from ctl->nAL on count as zero,
+// out[ctl->nAL] = total, and nothing runs when ctl->skip.
 template<class T>
-void scanExclusive(const T* in, T* out, int n, DevBuf& scratch, unsigned long long* total64, hipStream_t st);
+void scanExclusive(const T* in, T* out, int n, DevBuf& scratch, unsigned long long* total64, hipStream_t st,
+                   const PassCtl* ctl = nullptr);
 
 // Stable ascending sort of (keys, vals) over bits [0, bits).  Returns true when the
 // sorted result sits in keysAlt / valsAlt rather than keys / vals.  Host-synchronising

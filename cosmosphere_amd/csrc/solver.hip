@@ -18,6 +18,7 @@
  * hold only closures are not counted as iterations.
  */
 #include "density.hpp"
+#include "loop.hpp"
 #include "solver.hpp"
 #include "tree.hpp"
 
@@ -102,7 +103,8 @@ static bool periodicOf(const State& s) { return !s.hBox.empty() && s.hBox[0].per
 static DensArgs densArgs(State& s, const int* leaves, int nLeaves, double pmass, double tolh, double hfact,
                          unsigned long long* counters)
 {
-    DensArgs a{};
+    DensArgs a;
+    std::memset(&a, 0, sizeof(a));   // padding too: the device-driven loop keys its graph on the bytes
     a.leaves = leaves;
     a.nLeaves = nLeaves;
     a.jOffset = s.jOffset.as<uint32_t>();
@@ -180,10 +182,8 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         unsigned int nNext = 0;
         if (nIn > 0) HIP_CHECK(hipMemcpyAsync(&nNext, pos + nIn, sizeof(nNext), hipMemcpyDeviceToHost, st));
         readCounters(s, c);   // synchronises
-        std::swap(leaves, leavesAlt);
+        std::swap(leaves, leavesAlt);   // the buffers keep their places: the same two each solve
         std::swap(cnt, cntAlt);
-        std::swap(s.activeLeaves, s.activeLeavesAlt);
-        std::swap(s.itemCount, s.itemCountAlt);
         nAL = (int)nNext;
     };
 
@@ -193,6 +193,33 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
     unsigned long long nIterating = c[kCtrIterating], nItems = c[kCtrItems];
 
     Ev a0, a1, a2, a3;
+    bool devLoop = false;
+    if (mode == 0 && s.loopMode == 1 && nAL > 0)
+    {
+        // device-driven passes (loop.hip): the same passes without a host round trip each
+        DeviceLoopArgs dl;
+        std::memset(&dl, 0, sizeof(dl));
+        dl.leaves = leaves;
+        dl.leavesAlt = leavesAlt;
+        dl.cnt = cnt;
+        dl.cntAlt = cntAlt;
+        dl.flag = flag;
+        dl.pos = pos;
+        dl.itemOff = itemOff;
+        dl.itemPos = itemPos;
+        dl.itemPart = itemPart;
+        dl.nAL = nAL;
+        dl.nIterating = nIterating;
+        dl.sparse = sparse;
+        dl.periodic = per;
+        s.hRemLeaf.get<double>((size_t)L + 1);   // sized before its pointer is taken
+        dl.da = densArgs(s, leaves, L, pmass, tolh, hfact, nullptr);
+        dl.da.hmaxLeaf = s.hRemLeaf.as<double>();
+        dl.da.closure = closure && !withGradients;
+        nIterating = (unsigned long long)deviceLoop(s, dl, t);
+        nAL = 0;
+        devLoop = true;
+    }
     while (nAL > 0)
     {
         const bool nrPass = nIterating > 0;   // else: closures only
@@ -242,6 +269,7 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         nIterating = c[kCtrIterating];
         nItems = c[kCtrItems];
     }
+    (void)devLoop;
     const unsigned long long nActive = na > 0 ? nIterating : 0;
 
     int hovf[2] = {0, 0};

@@ -51,6 +51,7 @@ struct Timings
     double pairsAccepted = 0, pairsCandidate = 0;   // summed over NR iterations
     double kernLaunches = 0;                         // NR pair-kernel launches
     double closurePairs = 0, closureCand = 0;
+    double graphReplays = 0, graphCaptures = 0;      // device-driven loop
 };
 
 // One NR pass: active leaves, work items, particles iterating at its start, NR candidates and
@@ -135,6 +136,9 @@ struct State
     DevBuf counters;          // u64 device counters (pairs, candidates, unconverged)
     DevBuf pinnedCounters;    // host-visible copy target
     DevBuf ovf;               // int[2] list truncation / stack drop (never expected)
+    DevBuf passCtl;           // device-driven loop: control block, pass records, work queue order
+    DevBuf passLogDev;
+    DevBuf qOrder;
 
     // ---- force prep ----
     DevBuf hsqinv, hinv, rhoh, rho1, grkfac, pres, auterm, divfac;
@@ -148,6 +152,13 @@ struct State
     int      mapping = kMapAuto;   // auto: whole-leaf items, per-particle below `sparse`
     int      sparse = 16;          // per-particle items when at most this many remain in a leaf
     int      tailLeaves = 2048;    // ... and at most this many leaves are active
+    int      loopMode = 1;         // NR passes: 0 host-driven, 1 device-driven (FLAT_PARTICLE)
+    int      useGraph = 1;         // device-driven passes replayed as a captured HIP graph
+    int      graphPasses = 4;      // passes per graph replay (per host round trip)
+    int      lpt = 1;              // work queue in cost-descending order
+    int      loopBlockScale = 4;   // persistent density grid: resident blocks x scale / 4
+    int      loopBlocks = 0, loopBlocksPrecision = -1, loopBlocksPeriodic = -1;
+    long long loopListCap = 0;     // verification: starting list capacity of the device-driven loop (0: auto)
     std::vector<PassRecord> passLog;   // one record per NR pass of the last solve
     int      closureOn = -1;   // -1: on for mixed, off for plain
     int      treeOnly = 0;     // verification: stop after the tree build

@@ -9,6 +9,8 @@
 namespace csph
 {
 
+struct PassCtl;
+
 // Box, keys, sort, permutation (x, y, z, h and, if withVelAcc, v and a), leaf tree, node
 // geometry, layout and particle -> leaf map for the sets described in s.
 // periodicBox: nullptr (open, fitted boxes) or 6 doubles per set.
@@ -16,7 +18,8 @@ void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch
 
 // hmax of the given leaves (leaves == nullptr: all of them); with `state`, also the max h of
 // their particles in state != 0 into s.hRemLeaf (the reach of tight lists).
-void hmaxLeaves(State& s, const int* leaves, int nLeaves, const uint8_t* state = nullptr);
+// With ctl (device-driven pass) nLeaves is the capacity and ctl->nAL the count.
+void hmaxLeaves(State& s, const int* leaves, int nLeaves, const uint8_t* state = nullptr, const PassCtl* ctl = nullptr);
 
 // hmax per node, leaves first then internal levels deepest first (force upsweep).
 void hmaxUpsweep(State& s);
@@ -26,5 +29,11 @@ void hmaxUpsweep(State& s);
 // total number of entries.  tight: gather lists trimmed to s.hRemLeaf (same order, same
 // accepted pairs for the particles in state != 0).
 long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric, bool tight = false);
+
+// Device-driven pass: the count walk (jList == nullptr: counts into jCount) or the fill walk
+// (into jList at jOffset) of the tight gather lists of ctl->nAL leaves; nothing when
+// ctl->skip.  nCap bounds ctl->nAL.
+void launchGatherListsDevice(State& s, const int* leaves, int nCap, uint32_t* jCount, const uint32_t* jOffset,
+                             int* jList, const PassCtl* ctl);
 
 } // namespace csph

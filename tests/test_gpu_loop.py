@@ -1,0 +1,118 @@
+"""GPU parity of the device-driven Newton-Raphson loop (loop.hip) against the host-driven one.
+
+Both loops run the same passes with the same work items, lists and sums; the device-driven
+loop only changes who decides (the control block instead of the host) and the order in which
+wavefronts take items (a cost-ordered queue).  So h, rho and d(rho)/d(h) must be bit-identical,
+the iteration count equal and the pair counters equal, whether the passes are replayed as a
+HIP graph or launched directly, K passes per round trip, with or without the cost order, and
+when the list buffer overflows mid-loop (the pass stops, the buffer grows, the graph is
+captured again).  The host-driven loop is itself pinned to the FP64 oracle by
+test_gpu_parity.py and test_gpu_items.py; one oracle check is repeated here."""
+import numpy as np
+import pytest
+
+from cosmosphere_amd import Library
+from gpu_common import DEFAULT_ITEMS, dataset, gpu_run, oracle_run, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _reset_options():
+    yield
+    for kernel in ("cubic", "quintic"):
+        lib = Library(kernel)
+        for k, v in DEFAULT_ITEMS.items():
+            lib.set_option(k, v)
+
+
+def solve(c, precision, **opts):
+    lib, iters = gpu_run(c, precision=precision, gradients=False, **opts)
+    out = {k: lib.array(k).copy() for k in ("h", "rho", "gradh")}
+    out["iters"] = iters
+    for k in ("pairs", "candidates", "closure_pairs", "graph_captures", "graph_replays"):
+        out[k] = lib.stat(k)
+    return out
+
+
+def same(a, b):
+    for k in ("h", "rho", "gradh"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["iters"] == b["iters"]
+    assert a["pairs"] == b["pairs"]
+    assert a["closure_pairs"] == b["closure_pairs"]
+
+
+VARIANTS = [("graph_k4", {}), ("graph_k1", {"graph_passes": 1}), ("graph_k3", {"graph_passes": 3}),
+            ("direct", {"graph": 0}), ("no_lpt", {"lpt": 0}), ("overflow", {"loop_list_cap": 64}),
+            ("flat", {"mapping": 2}), ("whole", {"mapping": 1})]
+
+
+@pytest.mark.parametrize("label,opts", VARIANTS, ids=[v[0] for v in VARIANTS])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["plummer_cubic", "plummer_quintic", "dead", "C1small"])
+def test_device_loop_bit_identical(name, precision, label, opts):
+    c = dataset(name)
+    print("case", name, precision, label, flush=True)
+    base_opts = {k: v for k, v in opts.items() if k in ("mapping",)}
+    host = solve(c, precision, loop=0, **base_opts)
+    dev = solve(c, precision, loop=1, **opts)
+    same(dev, host)
+    assert dev["graph_replays"] >= 1
+    if label == "overflow":
+        assert dev["graph_captures"] >= 1
+
+
+def test_device_loop_oracle_and_graph_reuse():
+    c = dataset("plummer_quintic")
+    lib, iters = gpu_run(c, precision=0, gradients=False)
+    o, unconv = oracle_run(c, gradients=False)
+    assert unconv == 0 and iters == o.geti("itersRun")
+    na = o.geti("nAlive")
+    assert rel_err(lib.array("h")[:na], o.get("h")[:na]) < 1e-10
+    assert rel_err(lib.array("rho")[:na], o.get("rho")[:na]) < 1e-10
+    # repeated solves of the same inputs replay cached graphs: the particle arrays alternate
+    # between two buffers from one solve to the next, so at most one more capture, then none
+    h1 = lib.array("h").copy()
+    lib2, iters2 = gpu_run(c, precision=0, gradients=False)
+    caps = lib2.stat("graph_captures")
+    for _ in range(3):
+        lib3, iters3 = gpu_run(c, precision=0, gradients=False)
+        assert iters3 == iters
+        assert np.array_equal(lib3.array("h"), h1)
+    assert lib3.stat("graph_captures") == caps
+
+
+def test_device_loop_unconverged_cap():
+    # the NR pass cap ends the device loop exactly where the host loop stops
+    c = dataset("plummer_cubic")
+    host = solve(c, 0, loop=0, max_iter=100)
+    assert host["iters"] > 3
+    import subprocess
+    import sys
+    import os
+    code = ("import sys; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+            "from gpu_common import dataset, gpu_run\n"
+            "c = dataset('plummer_cubic')\n"
+            "gpu_run(c, precision=0, gradients=False, max_iter=3, loop=1)\n") % (
+        os.path.dirname(os.path.dirname(os.path.abspath(__file__))), os.path.dirname(os.path.abspath(__file__)))
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert res.returncode != 0
+    assert "FATAL: solveDensH: unconverged=" in res.stderr and "iters=3" in res.stderr
+
+
+def test_fresh_process_first_solve_mixed():
+    # regression: the first solve of a process, mixed precision, device-driven loop (the
+    # density arguments once took a buffer's pointer before the loop first sized it)
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    code = ("import sys; sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+            "from gpu_common import dataset, gpu_run\n"
+            "c = dataset('plummer_quintic')\n"
+            "lib, it = gpu_run(c, precision=1, gradients=False, loop=1)\n"
+            "print('iters', it, lib.stat('graph_replays'))\n") % (os.path.dirname(here), here)
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr[-2000:]
+    assert "iters" in res.stdout
