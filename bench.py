@@ -217,9 +217,18 @@ def bench_ensemble(args):
     lib.stash_inputs()
     pm = [s["pmass"] for s in sets]
     out = {}
+    tuned = {}
     for name, prec in (("fp64", 0), ("mixed", 1)):
         lib.set_option("precision", prec)
         lib.set_option("closure", -1)
+        for k, v in args.opts:
+            lib.set_option(k, v)
+        if args.tune_budget > 0:
+            # untimed: the autotuner times complete batched solves of these inputs and leaves the
+            # fastest options in force (the inputs are put back afterwards)
+            lib.restore_inputs()
+            tuned[name] = lib.batch_autotune(pm, tolh=1e-4, hfact=1.2, budget_s=args.tune_budget)
+            log("  autotune %s: %s" % (name, tuned[name]))
         for _ in range(args.warmup):
             lib.restore_inputs()
             lib.batch_solve(pm, 1e-4, 1.2)
@@ -253,6 +262,7 @@ def bench_ensemble(args):
         "sequential_fp64": {"ms": 1e3 * seq, "h_solves_per_s": sum(counts) / seq},
         "batched_fp64": out["fp64"],
         "batched_mixed_vs_sequential_fp64": m["h_solves_per_s"] / (sum(counts) / seq),
+        "autotune": tuned,
         "roofline": {"bound": "valu", "achieved": m["kernel_tflops"], "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": m["kernel_tflops"] / FP32_PEAK, "traffic": None, "kernel_avg_ms": m["kernel_avg_ms"]},
     }
@@ -268,10 +278,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--tune-budget", type=float, default=None,
+                    help="seconds of autotuning before the timed steps (default: 60 for C4, 0 otherwise)")
     ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
                     help="library option (cosmo_ext_set_option), e.g. loop=0 for the host-driven NR loop")
     args = ap.parse_args()
     args.opts = [(kv.split("=")[0], float(kv.split("=")[1])) for kv in args.opt]
+    if args.tune_budget is None:
+        args.tune_budget = 60.0 if args.config == "C4" else 0.0
     if args.gpus != 1:
         print("bench.py measures one GPU: this solver runs a particle set (or an ensemble of sets) on one MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)

@@ -30,7 +30,8 @@ ABI_SYMBOLS = ["densityiterate_gpu_c", "cosmo_kernel_radius", "force_gpu_c", "co
                "cosmo_pin_host", "cosmo_unpin_host"]
 EXT_SYMBOLS = ["cosmo_ext_solve_dens_h", "cosmo_ext_set_option", "cosmo_ext_get_stat",
                "cosmo_ext_get_array", "cosmo_ext_stash_inputs", "cosmo_ext_restore_inputs",
-               "cosmo_ext_sync", "cosmo_batch_init", "cosmo_batch_solve"]
+               "cosmo_ext_sync", "cosmo_batch_init", "cosmo_batch_solve", "cosmo_ext_autotune",
+               "cosmo_batch_autotune"]
 
 _ARRAY_DTYPES = {"keys": np.uint64, "order": np.int32, "leafStart": np.uint64, "leafLevel": np.int32,
                  "leafSet": np.int32, "layout": np.uint32, "particleLeaf": np.int32,
@@ -71,6 +72,11 @@ def _bind(L):
     L.cosmo_batch_init.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.cosmo_batch_solve.argtypes = [c_dp, ctypes.c_int, c_dp, ctypes.c_double, ctypes.c_double]
     L.cosmo_batch_solve.restype = ctypes.c_int
+    L.cosmo_ext_autotune.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, c_dp, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double]
+    L.cosmo_ext_autotune.restype = ctypes.c_int
+    L.cosmo_batch_autotune.argtypes = [c_dp, ctypes.c_int, c_dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    L.cosmo_batch_autotune.restype = ctypes.c_int
     return L
 
 
@@ -188,6 +194,29 @@ class Library:
         bx = np.ascontiguousarray(periodic_boxes if periodic_boxes is not None else np.zeros(6), dtype=np.float64)
         return self.L.cosmo_batch_solve(_dptr(pm), int(periodic_boxes is not None), _dptr(bx), float(tolh),
                                         float(hfact))
+
+    # ---- autotuner ----
+    TUNE_OPTIONS = ("loop", "bucket", "mapping", "sparse", "tail_leaves", "loop_blocks_scale", "graph_passes", "lpt")
+
+    def autotune(self, n, pmass, tolh=1e-4, hfact=-1.0, periodic_box=None, budget_s=60.0):
+        """Pick the solve's speed options for the uploaded set; returns the options chosen."""
+        box = np.ascontiguousarray(periodic_box if periodic_box is not None else np.zeros(6), dtype=np.float64)
+        self.L.cosmo_ext_autotune(int(n), float(pmass), int(periodic_box is not None), _dptr(box), float(tolh),
+                                  float(hfact), float(budget_s))
+        return self.tuned()
+
+    def batch_autotune(self, pmass, tolh=1e-4, hfact=-1.0, periodic_boxes=None, budget_s=60.0):
+        pm = np.ascontiguousarray(pmass, dtype=np.float64)
+        bx = np.ascontiguousarray(periodic_boxes if periodic_boxes is not None else np.zeros(6), dtype=np.float64)
+        self.L.cosmo_batch_autotune(_dptr(pm), int(periodic_boxes is not None), _dptr(bx), float(tolh), float(hfact),
+                                    float(budget_s))
+        return self.tuned()
+
+    def tuned(self):
+        out = {k: int(self.stat(k)) for k in self.TUNE_OPTIONS}
+        for k in ("tune_trials", "tune_best_s", "tune_default_s", "tune_seconds", "tune_cached"):
+            out[k] = self.stat(k)
+        return out
 
     # ---- convenience: the Phantom call sequence of the reference driver (src/main.cu:118-141) ----
     def solve(self, x, y, z, h, pmass, tolh=1e-4, hfact=-1.0, v=None, a=None, gradients=True, periodic_box=None,

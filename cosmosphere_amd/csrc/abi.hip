@@ -9,6 +9,7 @@
 #include "density.hpp"
 #include "solver.hpp"
 #include "tree.hpp"
+#include "tune.hpp"
 
 #include <chrono>
 #include <cstring>
@@ -328,6 +329,11 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "tail_leaves") s.tailLeaves = (int)value;
     else if (nm == "closure") s.closureOn = (int)value;
     else if (nm == "tree_only") s.treeOnly = (int)value;
+    else if (nm == "bucket")
+    {
+        if (!(value >= 8 && value <= 1024)) fatal("cosmo_ext_set_option: bucket must lie in [8, 1024]");
+        s.bucket = (int)value;
+    }
     else if (nm == "loop") s.loopMode = (int)value;
     else if (nm == "graph") s.useGraph = (int)value;
     else if (nm == "graph_passes") s.graphPasses = std::max(1, (int)value);
@@ -340,6 +346,8 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
         std::abort();
     }
 }
+
+double tuneStat(const std::string& nm);
 
 extern "C" double cosmo_ext_get_stat(const char* name)
 {
@@ -383,6 +391,8 @@ extern "C" double cosmo_ext_get_stat(const char* name)
     if (nm == "graph_replays") return t.graphReplays;
     if (nm == "graph_captures") return t.graphCaptures;
     if (nm == "loop") return s.loopMode;
+    if (nm == "bucket") return s.bucket;
+    if (nm.compare(0, 5, "tune_") == 0) return tuneStat(nm);
     if (nm == "graph") return s.useGraph;
     if (nm == "graph_passes") return s.graphPasses;
     if (nm == "lpt") return s.lpt;
@@ -503,7 +513,14 @@ extern "C" void cosmo_batch_init(int nSets, const int* counts)
     sizeParticleArrays(state(), (int)total);
 }
 
+static int batchSolve(const double* pmass, int periodic, const double* boxes, double tolh, double hfact);
+
 extern "C" int cosmo_batch_solve(const double* pmass, int periodic, const double* boxes, double tolh, double hfact)
+{
+    return batchSolve(pmass, periodic, boxes, tolh, hfact);
+}
+
+static int batchSolve(const double* pmass, int periodic, const double* boxes, double tolh, double hfact)
 {
     State& s = state();
     if (gBatchCounts.empty()) fatal("cosmo_batch_solve: cosmo_batch_init has not been called");
@@ -529,4 +546,67 @@ extern "C" int cosmo_batch_solve(const double* pmass, int periodic, const double
 extern "C" void cosmo_ext_sync(void)
 {
     if (state().stream) HIP_CHECK(hipStreamSynchronize(state().stream));
+}
+
+// =========================================================================================
+// Autotuner (tune.hip): picks the speed options of the solve for the uploaded workload.
+// =========================================================================================
+namespace
+{
+TuneResult gTune;
+
+std::string signatureOf(const char* kind, const State& s, int n, int periodic, double tolh, double hfact,
+                        const std::vector<int>& counts)
+{
+    unsigned long long h = 1469598103934665603ull;   // FNV-1a of the set sizes
+    for (int c : counts)
+        for (int k = 0; k < 4; ++k)
+        {
+            h ^= (unsigned long long)((c >> (8 * k)) & 0xFF);
+            h *= 1099511628211ull;
+        }
+    char b[256];
+    std::snprintf(b, sizeof(b), "%s n=%d sets=%zu sizes=%016llx R=%g precision=%d periodic=%d tolh=%g hfact=%g "
+                                "closure=%d max_iter=%d",
+                  kind, n, counts.size(), h, kRad, s.precision, periodic, tolh, hfact, s.closureOn, s.maxIter);
+    return b;
+}
+} // namespace
+
+extern "C" int cosmo_ext_autotune(int n, double pmass, int periodic, const double* box, double tolh, double hfact,
+                                  double budgetSeconds)
+{
+    State& s = state();
+    requireSized(s, n, "cosmo_ext_autotune");
+    ensureStream(s);
+    const std::string sig = signatureOf("single", s, n, periodic, tolh, hfact, std::vector<int>{n});
+    gTune = autotune(s, sig, [&]() {
+        setSingle(s, n);
+        solveDensH(s, pmass, 0, false, periodic ? box : nullptr, tolh, hfact);
+    }, budgetSeconds);
+    return gTune.trials;
+}
+
+extern "C" int cosmo_batch_autotune(const double* pmass, int periodic, const double* boxes, double tolh, double hfact,
+                                    double budgetSeconds)
+{
+    State& s = state();
+    if (gBatchCounts.empty()) fatal("cosmo_batch_autotune: cosmo_batch_init has not been called");
+    long long total = 0;
+    for (int c : gBatchCounts) total += c;
+    requireSized(s, (int)total, "cosmo_batch_autotune");
+    ensureStream(s);
+    const std::string sig = signatureOf("batch", s, (int)total, periodic, tolh, hfact, gBatchCounts);
+    gTune = autotune(s, sig, [&]() { batchSolve(pmass, periodic, boxes, tolh, hfact); }, budgetSeconds);
+    return gTune.trials;
+}
+
+double tuneStat(const std::string& nm)
+{
+    if (nm == "tune_trials") return gTune.trials;
+    if (nm == "tune_best_s") return gTune.bestSeconds;
+    if (nm == "tune_default_s") return gTune.defaultSeconds;
+    if (nm == "tune_seconds") return gTune.tuneSeconds;
+    if (nm == "tune_cached") return gTune.cached ? 1.0 : 0.0;
+    return -1.0;
 }

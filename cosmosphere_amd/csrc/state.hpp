@@ -152,6 +152,7 @@ struct State
     int      mapping = kMapAuto;   // auto: whole-leaf items, per-particle below `sparse`
     int      sparse = 16;          // per-particle items when at most this many remain in a leaf
     int      tailLeaves = 2048;    // ... and at most this many leaves are active
+    int      bucket = kBucket;     // leaf bucket size (64: the reference's; others are speed options)
     int      loopMode = 1;         // NR passes: 0 host-driven, 1 device-driven (FLAT_PARTICLE)
     int      useGraph = 1;         // device-driven passes replayed as a captured HIP graph
     int      graphPasses = 4;      // passes per graph replay (per host round trip)

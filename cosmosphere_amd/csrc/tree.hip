@@ -211,14 +211,16 @@ struct WinInfo
     int e, cmax, b;
 };
 
+// bucket: a node holding more than `bucket` keys is split (64 is the reference's bucket and
+// the size at which tree and lists are compared bit for bit; other sizes are speed options).
 __device__ inline WinInfo windowOf(const uint64_t* __restrict__ keys, int i, const int* __restrict__ aliveFirst,
-                                   const int* __restrict__ alive, int nSets)
+                                   const int* __restrict__ alive, int nSets, int bucket)
 {
     WinInfo w;
     w.b = nSets > 1 ? setOf(aliveFirst, nSets, i) : 0;
     const int q = i - aliveFirst[w.b], nb = alive[w.b];
     const uint64_t k = keys[i];
-    const int c = (q + kBucket < nb) ? commonDigits(k, keys[i + kBucket]) : -1;
+    const int c = (q + bucket < nb) ? commonDigits(k, keys[i + bucket]) : -1;
     w.e    = (q > 0) ? commonDigits(keys[i - 1], k) : -1;
     w.cmax = c < kMaxLevel - 1 ? c : kMaxLevel - 1;
     return w;
@@ -226,18 +228,18 @@ __device__ inline WinInfo windowOf(const uint64_t* __restrict__ keys, int i, con
 
 __global__ __launch_bounds__(256) void windowCountKernel(const uint64_t* __restrict__ keys, int nAliveTot,
                                                          const int* __restrict__ aliveFirst,
-                                                         const int* __restrict__ alive, int nSets,
+                                                         const int* __restrict__ alive, int nSets, int bucket,
                                                          uint32_t* __restrict__ cnt)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nAliveTot) return;
-    const WinInfo w = windowOf(keys, i, aliveFirst, alive, nSets);
+    const WinInfo w = windowOf(keys, i, aliveFirst, alive, nSets, bucket);
     cnt[i] = (w.cmax > w.e) ? (uint32_t)(w.cmax - w.e) : 0u;
 }
 
 __global__ __launch_bounds__(256) void windowEmitKernel(const uint64_t* __restrict__ keys, int nAliveTot,
                                                         const int* __restrict__ aliveFirst,
-                                                        const int* __restrict__ alive, int nSets,
+                                                        const int* __restrict__ alive, int nSets, int bucket,
                                                         const uint32_t* __restrict__ off,
                                                         uint64_t* __restrict__ preStart, int* __restrict__ preLevel,
                                                         int* __restrict__ preSet, uint64_t* __restrict__ sortKey,
@@ -245,7 +247,7 @@ __global__ __launch_bounds__(256) void windowEmitKernel(const uint64_t* __restri
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nAliveTot) return;
-    const WinInfo w = windowOf(keys, i, aliveFirst, alive, nSets);
+    const WinInfo w = windowOf(keys, i, aliveFirst, alive, nSets, bucket);
     const uint64_t k = keys[i];
     uint32_t o = off[i];
     for (int l = w.e + 1; l <= w.cmax; ++l, ++o)
@@ -615,7 +617,8 @@ void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch
     uint32_t* woff = s.winOffset.get<uint32_t>(na + 2);
     if (na > 0)
     {
-        windowCountKernel<<<iceil(na, 256), 256, 0, st>>>(keys, na, s.aliveFirst.as<int>(), s.alive.as<int>(), B, wcnt);
+        windowCountKernel<<<iceil(na, 256), 256, 0, st>>>(keys, na, s.aliveFirst.as<int>(), s.alive.as<int>(), B,
+                                                           s.bucket, wcnt);
         LAUNCH_CHECK();
     }
     scanExclusive<uint32_t>(wcnt, woff, na, s.scanTmp, nullptr, st);
@@ -633,8 +636,8 @@ void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch
     uint32_t* biA = s.bfsIdxAlt.get<uint32_t>(nInt + 1);
     if (na > 0)
     {
-        windowEmitKernel<<<iceil(na, 256), 256, 0, st>>>(keys, na, s.aliveFirst.as<int>(), s.alive.as<int>(), B, woff,
-                                                         preStart, preLevel, preSet, bk, bi);
+        windowEmitKernel<<<iceil(na, 256), 256, 0, st>>>(keys, na, s.aliveFirst.as<int>(), s.alive.as<int>(), B,
+                                                         s.bucket, woff, preStart, preLevel, preSet, bk, bi);
         LAUNCH_CHECK();
     }
     if (radixSortPairs(bk, bkA, bi, biA, nInt, 64, sc, st)) { std::swap(bk, bkA); std::swap(bi, biA); }

@@ -1,0 +1,231 @@
+/*
+ * tune.hip -- autotuner of the smoothing-length solve.
+ *
+ * The fastest shape of the NR passes depends on the workload: a uniform lattice of 1.4M
+ * particles runs a few short passes where launch and synchronisation costs matter, a
+ * clustered 8M set runs twenty passes dominated by long lists in dense spheres, an ensemble of
+ * hundreds of small sets sits between.  The tuner times complete solves of the caller's own
+ * inputs over the options the results do not depend on:
+ *
+ *   loop          host-driven passes, or device-driven passes (graph-replayed, persistent grid)
+ *   bucket        leaf bucket size (leaf size vs list length)
+ *   mapping       wave per leaf, wave per particle, or the automatic switch (sparse, tailLeaves)
+ *   blockScale    persistent grid size (resident blocks, or oversubscribed)
+ *   graphPasses   passes per graph replay (idle passes at the end vs host round trips)
+ *   lpt           cost-ordered work queue
+ *
+ * by a coordinate search: one option at a time, each candidate value timed by a solve (a second
+ * solve confirms a candidate within 5% of the best), the best kept, sweeps repeated until
+ * one changes nothing or the time budget is spent.  Results are remembered per workload
+ * signature for the life of the process, and in the file named by COSMO_TUNE_CACHE if set.
+ */
+#include "tune.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <vector>
+
+namespace csph
+{
+
+TuneParams currentParams(const State& s)
+{
+    return TuneParams{s.loopMode, s.bucket, s.mapping, s.sparse, s.tailLeaves, s.loopBlockScale, s.graphPasses, s.lpt};
+}
+
+void applyParams(State& s, const TuneParams& p)
+{
+    s.loopMode = p.loop;
+    s.bucket = p.bucket;
+    s.mapping = p.mapping;
+    s.sparse = p.sparse;
+    s.tailLeaves = p.tailLeaves;
+    s.loopBlockScale = p.blockScale;
+    s.graphPasses = p.graphPasses;
+    s.lpt = p.lpt;
+}
+
+std::string describe(const TuneParams& p)
+{
+    char b[256];
+    std::snprintf(b, sizeof(b), "loop=%d bucket=%d mapping=%d sparse=%d tail_leaves=%d loop_blocks_scale=%d "
+                                "graph_passes=%d lpt=%d",
+                  p.loop, p.bucket, p.mapping, p.sparse, p.tailLeaves, p.blockScale, p.graphPasses, p.lpt);
+    return b;
+}
+
+static bool parse(const std::string& text, TuneParams& p)
+{
+    return std::sscanf(text.c_str(), "loop=%d bucket=%d mapping=%d sparse=%d tail_leaves=%d loop_blocks_scale=%d "
+                                     "graph_passes=%d lpt=%d",
+                       &p.loop, &p.bucket, &p.mapping, &p.sparse, &p.tailLeaves, &p.blockScale, &p.graphPasses,
+                       &p.lpt) == 8;
+}
+
+namespace
+{
+
+std::map<std::string, TuneParams>& memo()
+{
+    static auto* m = new std::map<std::string, TuneParams>();
+    return *m;
+}
+
+// COSMO_TUNE_CACHE: one line per signature, "signature<TAB>options".
+void loadFileCache()
+{
+    static bool loaded = false;
+    if (loaded) return;
+    loaded = true;
+    const char* path = std::getenv("COSMO_TUNE_CACHE");
+    if (!path) return;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line))
+    {
+        const size_t tab = line.find('\t');
+        TuneParams p;
+        if (tab != std::string::npos && parse(line.substr(tab + 1), p)) memo()[line.substr(0, tab)] = p;
+    }
+}
+
+void storeFileCache(const std::string& sig, const TuneParams& p)
+{
+    const char* path = std::getenv("COSMO_TUNE_CACHE");
+    if (!path) return;
+    std::ofstream f(path, std::ios::app);
+    f << sig << '\t' << describe(p) << '\n';
+}
+
+// Device copy of the inputs as uploaded (caller order): x, y, z, h, v, a.
+struct Inputs
+{
+    State& s;
+    DevBuf keep;
+    int n;
+    explicit Inputs(State& st) : s(st), n(st.sizedFor) {}
+    DevBuf* field(int k)
+    {
+        DevBuf* f[10] = {&s.x, &s.y, &s.z, &s.h, &s.vx, &s.vy, &s.vz, &s.ax, &s.ay, &s.az};
+        return f[k];
+    }
+    void save()
+    {
+        double* d = keep.get<double>((size_t)10 * n + 1);
+        for (int k = 0; k < 10; ++k)
+            HIP_CHECK(hipMemcpyAsync(d + (size_t)k * n, field(k)->as<double>(), (size_t)n * 8, hipMemcpyDeviceToDevice,
+                                     s.stream));
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+    }
+    void restore()
+    {
+        const double* d = keep.as<double>();
+        for (int k = 0; k < 10; ++k)
+            HIP_CHECK(hipMemcpyAsync(field(k)->as<double>(), d + (size_t)k * n, (size_t)n * 8, hipMemcpyDeviceToDevice,
+                                     s.stream));
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+    }
+    ~Inputs()
+    {
+        if (keep.p) (void)hipFree(keep.p);
+    }
+};
+
+struct Dim
+{
+    int TuneParams::*field;
+    std::vector<int> values;
+    bool (*relevant)(const TuneParams&);
+};
+
+bool always(const TuneParams&) { return true; }
+bool deviceLoopOn(const TuneParams& p) { return p.loop == 1; }
+bool autoMapping(const TuneParams& p) { return p.mapping == 0; }
+
+} // namespace
+
+TuneResult autotune(State& s, const std::string& signature, const std::function<void()>& solve,
+                    double budgetSeconds)
+{
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto elapsed = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    TuneResult r;
+    loadFileCache();
+    const TuneParams start = currentParams(s);
+    auto it = memo().find(signature);
+    if (it != memo().end())
+    {
+        applyParams(s, it->second);
+        r.best = it->second;
+        r.cached = true;
+        return r;
+    }
+
+    Inputs in(s);
+    in.save();
+    auto timeOnce = [&](const TuneParams& p) {
+        applyParams(s, p);
+        in.restore();
+        const auto a = clk::now();
+        solve();
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        ++r.trials;
+        return std::chrono::duration<double>(clk::now() - a).count();
+    };
+
+    // a first solve of the starting options warms the caches and allocations; the second is timed
+    (void)timeOnce(start);
+    TuneParams best = start;
+    double bestT = timeOnce(start);
+    r.defaultSeconds = bestT;
+
+    const std::vector<Dim> dims = {
+        {&TuneParams::loop, {1, 0}, always},
+        {&TuneParams::bucket, {64, 32, 128}, always},
+        {&TuneParams::mapping, {0, 1, 2}, always},
+        {&TuneParams::sparse, {4, 8, 16, 32, 64}, autoMapping},
+        {&TuneParams::tailLeaves, {512, 2048, 8192, 32768}, autoMapping},
+        {&TuneParams::blockScale, {4, 8, 2}, deviceLoopOn},
+        {&TuneParams::graphPasses, {4, 1, 2, 8}, deviceLoopOn},
+        {&TuneParams::lpt, {1, 0}, deviceLoopOn},
+    };
+    bool improved = true;
+    for (int sweep = 0; sweep < 3 && improved && elapsed() < budgetSeconds; ++sweep)
+    {
+        improved = false;
+        for (const Dim& d : dims)
+        {
+            if (!d.relevant(best)) continue;
+            for (int v : d.values)
+            {
+                if (elapsed() >= budgetSeconds) break;
+                if (best.*(d.field) == v) continue;
+                TuneParams c = best;
+                c.*(d.field) = v;
+                double t = timeOnce(c);
+                if (t < 1.05 * bestT) t = std::min(t, timeOnce(c));   // confirm a close candidate
+                if (t < 0.99 * bestT)
+                {
+                    best = c;
+                    bestT = t;
+                    improved = true;
+                }
+            }
+        }
+    }
+    applyParams(s, best);
+    in.restore();
+    r.best = best;
+    r.bestSeconds = bestT;
+    r.tuneSeconds = elapsed();
+    memo()[signature] = best;
+    storeFileCache(signature, best);
+    return r;
+}
+
+} // namespace csph

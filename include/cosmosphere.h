@@ -91,6 +91,10 @@ int cosmo_ext_solve_dens_h(int n, double pmass, int mode, int withGradients, int
                            double tolh, double hfact);
 
 /* Options: "precision" (0 = plain FP64, 1 = mixed FP32-offset form), "max_iter",
+ * "loop" (NR passes: 0 host-driven, 1 device-driven, the default: counts in device memory,
+ * "graph_passes" passes per HIP graph replay when "graph" is 1, persistent density grid of
+ * "loop_blocks_scale" / 4 times the resident blocks, cost-ordered work queue when "lpt" is 1),
+ * "bucket" (leaf bucket size, default 64),
  * "mapping" (NR work items: 0 auto = one wavefront per leaf, one per particle once at most
  * "sparse" particles of the leaf remain; 1 = always one per leaf; 2 = always one per
  * particle), "sparse" (default 16), "closure" (-1 auto = on for mixed, 0 off, 1 on: rho and
@@ -122,6 +126,21 @@ void cosmo_ext_sync(void);
  * doubles per set when periodic != 0.  Returns the Newton iterations run. */
 void cosmo_batch_init(int nSets, const int* counts);
 int cosmo_batch_solve(const double* pmass, int periodic, const double* boxes, double tolh, double hfact);
+
+/* Autotuner: times complete density-only solves (FLAT_PARTICLE) of the uploaded inputs over the
+ * options the results do not depend on -- NR loop driver ("loop"), leaf bucket size ("bucket"),
+ * wave-per-leaf vs flat item mapping ("mapping", "sparse", "tail_leaves"), persistent grid size
+ * ("loop_blocks_scale"), passes per graph replay ("graph_passes"), cost-ordered queue ("lpt")
+ * -- one option at a time, and leaves the fastest in force.  The uploaded inputs are put back
+ * afterwards (a following solve starts from them; download nothing before it).  budgetSeconds
+ * bounds the search.  A workload with the same signature (particle count, set sizes, kernel,
+ * precision, tolerances) reuses the earlier choice (also across processes through the file named
+ * by COSMO_TUNE_CACHE).  Returns the trial solves run (0: reused).  The tree and lists are the
+ * reference's at bucket 64 only; h and rho do not depend on the bucket beyond summation order. */
+int cosmo_ext_autotune(int n, double pmass, int periodic, const double* box, double tolh, double hfact,
+                       double budgetSeconds);
+int cosmo_batch_autotune(const double* pmass, int periodic, const double* boxes, double tolh, double hfact,
+                         double budgetSeconds);
 
 #ifdef __cplusplus
 }

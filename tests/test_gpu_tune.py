@@ -1,0 +1,127 @@
+"""GPU tests of the speed options the autotuner chooses among, and of the autotuner itself.
+
+Leaf bucket size: at a bucket other than the reference's 64 the tree must still be the unique
+canonical tree "split iff more than `bucket` live keys and level < 21" -- bit-exact against the
+oracle built with the same bucket -- and the plain FP64 solve must reach the oracle's fixed
+point (h, rho within 1e-10, same iteration count), since h does not depend on how the particles
+are grouped into leaves beyond summation order.
+
+Autotuner: it runs trial solves, leaves options from its candidate sets in force, puts the
+uploaded inputs back (the next solve matches the oracle), and a second call with the same
+workload reuses the choice without trials."""
+import numpy as np
+import pytest
+
+from cosmosphere_amd import Library
+from cosmosphere_amd.api import HSML, POS
+from gpu_common import DEFAULT_ITEMS, dataset, gpu_run, oracle_run, rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TUNE_DEFAULTS = dict(DEFAULT_ITEMS, bucket=64, loop_blocks_scale=4, tail_leaves=2048)
+
+
+@pytest.fixture(autouse=True)
+def _reset_options():
+    yield
+    for kernel in ("cubic", "quintic"):
+        lib = Library(kernel)
+        for k, v in TUNE_DEFAULTS.items():
+            lib.set_option(k, v)
+
+
+@pytest.mark.parametrize("bucket", [32, 128, 16])
+@pytest.mark.parametrize("name", ["plummer_cubic", "plummer_quintic", "dead"])
+def test_bucket_tree_and_solve(name, bucket):
+    c = dataset(name)
+    print("case", name, "bucket", bucket, flush=True)
+    lib, _ = gpu_run(c, gradients=False, tree_only=True, bucket=bucket)
+    o = O.Oracle(quintic=c["quintic"], bucket=bucket)
+    o.set_particles(c["x"], c["y"], c["z"], c["h"])
+    o.build()
+    assert int(lib.stat("n_leaves")) == o.geti("nLeaves")
+    np.testing.assert_array_equal(lib.array("leafStart"), o.get("csTree")[:-1])
+    np.testing.assert_array_equal(lib.array("layout"), o.get("layout"))
+    np.testing.assert_array_equal(lib.array("intStart"), o.get("internalStart"))
+    for loop in (0, 1):
+        lib, iters = gpu_run(c, precision=0, gradients=False, bucket=bucket, loop=loop)
+        o2, unconv = oracle_run(c, gradients=False)
+        assert unconv == 0 and iters == o2.geti("itersRun")
+        na = o2.geti("nAlive")
+        assert rel_err(lib.array("h")[:na], o2.get("h")[:na]) < 1e-10
+        assert rel_err(lib.array("rho")[:na], o2.get("rho")[:na]) < 1e-10
+
+
+CANDIDATES = {"loop": {0, 1}, "bucket": {32, 64, 128}, "mapping": {0, 1, 2}, "sparse": {4, 8, 16, 32, 64},
+              "tail_leaves": {512, 2048, 8192, 32768}, "loop_blocks_scale": {2, 4, 8}, "graph_passes": {1, 2, 4, 8},
+              "lpt": {0, 1}}
+
+
+def test_autotune_single_set():
+    c = dataset("plummer_quintic")
+    lib = Library("quintic")
+    for k, v in TUNE_DEFAULTS.items():
+        lib.set_option(k, v)
+    lib.set_option("precision", 0)
+    lib.set_option("max_iter", 100)
+    lib.set_option("tree_only", 0)
+    n = c["n"]
+    lib.arrays_init(n)
+    lib.upload(POS, np.stack([c["x"], c["y"], c["z"]]))
+    lib.upload(HSML, c["h"])
+    lib.upload(3, np.zeros((3, n)))
+    lib.upload(4, np.zeros((3, n)))
+    got = lib.autotune(n, c["pmass"], tolh=1e-4, hfact=c["hfact"], budget_s=20.0)
+    print("tuned", got, flush=True)
+    assert got["tune_trials"] >= 2 and got["tune_cached"] == 0
+    for k, allowed in CANDIDATES.items():
+        assert got[k] in allowed, (k, got[k])
+    assert got["tune_best_s"] <= got["tune_default_s"] * 1.0001
+    # the inputs are back: the next solve (tuned options) reaches the oracle's fixed point
+    iters = lib.solve_dens_h(n, c["pmass"], tolh=1e-4, hfact=c["hfact"])
+    o, unconv = oracle_run(c, gradients=False)
+    assert unconv == 0 and iters == o.geti("itersRun")
+    na = o.geti("nAlive")
+    assert rel_err(lib.array("h")[:na], o.get("h")[:na]) < 1e-10
+    # same workload again: the choice is reused, no trial solves
+    lib.upload(POS, np.stack([c["x"], c["y"], c["z"]]))
+    lib.upload(HSML, c["h"])
+    again = lib.autotune(n, c["pmass"], tolh=1e-4, hfact=c["hfact"], budget_s=20.0)
+    assert again["tune_cached"] == 1 and again["tune_trials"] == 0
+    assert all(again[k] == got[k] for k in CANDIDATES)
+
+
+def test_autotune_batch():
+    from test_gpu_batch import ensemble
+    sets = ensemble()
+    lib = Library("cubic")
+    for k, v in TUNE_DEFAULTS.items():
+        lib.set_option(k, v)
+    lib.set_option("precision", 1)
+    lib.set_option("max_iter", 100)
+    lib.set_option("tree_only", 0)
+    n = lib.batch_init([s["n"] for s in sets])
+    lib.upload(POS, np.concatenate([np.stack([s["x"], s["y"], s["z"]]) for s in sets], axis=1))
+    lib.upload(HSML, np.concatenate([s["h"] for s in sets]))
+    lib.upload(3, np.zeros((3, n)))
+    lib.upload(4, np.zeros((3, n)))
+    pm = [s["pmass"] for s in sets]
+    got = lib.batch_autotune(pm, tolh=1e-4, hfact=1.2, budget_s=20.0)
+    print("tuned", got, flush=True)
+    assert got["tune_trials"] >= 2
+    lib.batch_solve(pm, tolh=1e-4, hfact=1.2)
+    h = lib.download(HSML, n)[0]
+    first = 0
+    for s in sets:
+        o = O.Oracle(quintic=False)
+        o.set_particles(s["x"], s["y"], s["z"], s["h"])
+        o.build()
+        assert o.density_solve(s["pmass"], 1e-4, 1.2, 100, 0, False) == 0
+        order, na = o.get("order"), o.geti("nAlive")
+        h_ref = np.empty(s["n"])
+        h_ref[order] = o.get("h")
+        live = np.zeros(s["n"], bool)
+        live[order[:na]] = True
+        assert rel_err(h[first:first + s["n"]][live], h_ref[live]) < 1e-6
+        first += s["n"]
