@@ -34,6 +34,10 @@ FP64_PEAK = 78.6    # TFLOP/s, MI355X vector FP64
 # Flop convention of the density pair sum (fixed): 9 per candidate pair (3 sub, r^2, scale),
 # 21 more per accepted pair (sqrt, kernel polynomial, accumulation).
 FLOPS_CAND, FLOPS_ACC = 9.0, 21.0
+# Flop convention of the force pair sum (fixed): 12 per candidate pair (separation, r^2, the two
+# support tests), 120 more per accepted pair (unit vector, signal speeds, viscosity, both kernel
+# gradients, the i- and j-side force, heating and div v terms)
+FORCE_FLOPS_CAND, FORCE_FLOPS_ACC = 12.0, 120.0
 # NR work-item options (the library's defaults)
 DEFAULT_ITEMS = {"mapping": 0, "sparse": 16, "tail_leaves": 2048}
 
@@ -269,6 +273,121 @@ def bench_ensemble(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_force(args, n_override=None):
+    """C5: density + force on the 16M-particle Plummer mixture with velocities, through the
+    reference's call sequence (densityiterate_gpu_c with gradients -> THERMO -> force_gpu_c), and
+    a loop of 10 drift steps x += v dt that carries h and the accelerations forward (11 solves).
+    Mixed vs plain FP64; per-output errors of the mixed form against the plain FP64 GPU path
+    (itself pinned to the FP64 oracle by the GPU parity tests) at steps 1 and 10."""
+    from cosmosphere_amd.api import DENS_OUT, FORCE_OUT, THERMO
+    t = time.perf_counter()
+    c = workloads.config5() if n_override is None else workloads.config5(n=n_override)
+    n, pmass, hfact, dt = c["n"], c["pmass"], c["hfact"], c["dt"]
+    nsteps = c["nsteps"] if args.force_steps is None else args.force_steps
+    log("workload %s: %d particles (generated in %.1f s)" % (c["name"], n, time.perf_counter() - t))
+    v = np.stack([c["vx"], c["vy"], c["vz"]])
+    gamma = c["gamma"]
+    lib = Library(c["kernel"])
+    lib.arrays_init(n)
+    keep_steps = (1, nsteps)
+    res, snaps = {}, {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        if args.only not in ("both", name):
+            continue
+        lib.set_option("precision", prec)
+        lib.set_option("max_iter", 100)
+        lib.set_option("closure", -1)
+        lib.set_option("tree_only", 0)
+        for k, val in args.opts:
+            lib.set_option(k, val)
+        x, y, z = c["x"].copy(), c["y"].copy(), c["z"].copy()
+        h = c["h"].copy()
+        a = np.zeros((3, n))
+        rec = []
+        snaps[name] = {}
+        for step in range(nsteps + 1):
+            t0 = time.perf_counter()
+            lib.upload(POS, np.stack([x, y, z]))
+            lib.upload(HSML, h)
+            lib.upload(VEL, v)
+            lib.upload(ACCEL, a)
+            lib.sync()
+            t1 = time.perf_counter()
+            lib.densityiterate(n, pmass, tolh=1e-4, hfact=hfact)
+            t2 = time.perf_counter()
+            dens = {k: lib.stat(k) for k in ("total", "iters", "nrkern", "gradkern", "gjbuild", "jbuild", "pairs",
+                                             "candidates")}
+            hnew = lib.download(HSML, n)[0]
+            rhoh = pmass * (hfact / hnew) ** 3
+            th = np.stack([(gamma - 1.0) * c["u"] / rhoh, c["cs"], c["alpha"], c["u"]])
+            lib.upload_sorted(THERMO, th)
+            lib.sync()
+            t3 = time.perf_counter()
+            lib.force(n, pmass, beta=c["beta"], alphau=c["alphau"], disc_viscosity=0, pdv_heating=1, shock_heating=1)
+            t4 = time.perf_counter()
+            frc = {k: lib.stat(k) for k in ("force_total", "force_kernel", "force_jbuild", "force_pairs",
+                                            "force_candidates")}
+            f = lib.download(FORCE_OUT, n)
+            t5 = time.perf_counter()
+            rec.append({"step": step, "density_s": t2 - t1, "force_s": t4 - t3, "step_wall_s": t5 - t0, **dens, **frc})
+            log("  %s step %d: density %.1f ms (%d iters), force %.1f ms (kernel %.1f ms), step with transfers "
+                "%.1f ms" % (name, step, 1e3 * (t2 - t1), dens["iters"], 1e3 * (t4 - t3), 1e3 * frc["force_kernel"],
+                             1e3 * (t5 - t0)))
+            if step in keep_steps:
+                d = lib.download(DENS_OUT, n)
+                snaps[name][step] = {"h": hnew.copy(), "rho": d[0].copy(), "f": f.copy()}
+            # drift, carrying h and the accelerations into the next solve
+            x, y, z = x + v[0] * dt, y + v[1] * dt, z + v[2] * dt
+            h = hnew
+            a = f[:3].copy()
+        tot_d = sum(r["density_s"] for r in rec)
+        tot_f = sum(r["force_s"] for r in rec)
+        tot_k = sum(r["force_kernel"] for r in rec)
+        fl = FORCE_FLOPS_CAND * sum(r["force_candidates"] for r in rec) + FORCE_FLOPS_ACC * sum(r["force_pairs"] for r in rec)
+        res[name] = {
+            "particle_updates_per_s": n * len(rec) / (tot_d + tot_f),
+            "ms_per_step": 1e3 * (tot_d + tot_f) / len(rec),
+            "ms_per_step_with_transfers": 1e3 * sum(r["step_wall_s"] for r in rec) / len(rec),
+            "density_ms_step0": 1e3 * rec[0]["density_s"],
+            "density_ms_carried": 1e3 * sum(r["density_s"] for r in rec[1:]) / max(len(rec) - 1, 1),
+            "iters_step0": rec[0]["iters"],
+            "iters_carried": [int(r["iters"]) for r in rec[1:]],
+            "gradient_sweep_ms": 1e3 * sum(r["gradkern"] for r in rec) / len(rec),
+            "gradient_lists_ms": 1e3 * sum(r["gjbuild"] for r in rec) / len(rec),
+            "nr_kernel_ms_carried": 1e3 * sum(r["nrkern"] for r in rec[1:]) / max(len(rec) - 1, 1),
+            "nr_lists_ms_carried": 1e3 * sum(r["jbuild"] for r in rec[1:]) / max(len(rec) - 1, 1),
+            "force_candidates_per_pair": sum(r["force_candidates"] for r in rec) / max(sum(r["force_pairs"] for r in rec), 1),
+            "force_ms": 1e3 * tot_f / len(rec),
+            "force_kernel_ms": 1e3 * tot_k / len(rec),
+            "force_pair_interactions_per_s": sum(r["force_pairs"] for r in rec) / max(tot_k, 1e-12),
+            "force_kernel_tflops": fl / max(tot_k, 1e-12) / 1e12,
+            "density_pair_interactions_per_s": sum(r["pairs"] for r in rec) / max(sum(r["nrkern"] for r in rec), 1e-12),
+        }
+    out = {"metric": METRIC + " (density + force)", "unit": "particle density+force updates/s", "n_gpus": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+           "config": {"workload": c["name"], "n": n, "kernel": c["kernel"], "solves": nsteps + 1,
+                      "dt": dt, "gamma": gamma, "alpha_AV": 1.0, "beta": c["beta"], "parallelism": "single GPU"}}
+    head = "mixed" if "mixed" in res else "fp64"
+    out.update({"value": res[head]["particle_updates_per_s"], "ms_per_step": res[head]["ms_per_step"],
+                "dtype": "fp32 pair sums + fp64 Newton" if head == "mixed" else "fp64", head: res[head]})
+    for k in res:
+        out[k] = res[k]
+    if "fp64" in res and "mixed" in res:
+        out["mixed_vs_fp64"] = res["mixed"]["particle_updates_per_s"] / res["fp64"]["particle_updates_per_s"]
+        err = {}
+        for step in keep_steps:
+            a_, b_ = snaps["mixed"][step], snaps["fp64"][step]
+            fmax = np.abs(b_["f"][:3]).max()
+            e = {"h": rel_stats(a_["h"], b_["h"]), "rho": rel_stats(a_["rho"], b_["rho"])}
+            for k, nm in enumerate(("fx", "fy", "fz", "dudt")):
+                scale = fmax if k < 3 else np.abs(b_["f"][3]).max()
+                d = np.abs(a_["f"][k] - b_["f"][k]) / scale
+                e[nm + "_rel_to_max"] = {"max": float(d.max()), "p99": float(np.percentile(d, 99))}
+            err["step%d" % step] = e
+        out["mixed_vs_fp64_error"] = err
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +397,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--force-steps", type=int, default=None, help="C5: drift steps after the first solve (default 10)")
     ap.add_argument("--tune-budget", type=float, default=None,
                     help="seconds of autotuning before the timed steps (default: 60 for C4, 0 otherwise)")
     ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
@@ -292,6 +412,9 @@ def main():
         sys.exit(2)
     if args.config == "C4":
         bench_ensemble(args)
+        return
+    if args.config == "C5" or args.config.startswith("C5:"):
+        bench_force(args, int(args.config[3:]) if args.config.startswith("C5:") else None)
         return
 
     t = time.perf_counter()

@@ -79,73 +79,114 @@ __device__ inline double xiLimiter(const double g[9])
     return (fac + tr > 2.220446049250313e-16) ? fac / (fac + tr) : 1.0;
 }
 
+// One wavefront per leaf, the leaf's particles up to 64 at a time.  With c particles in a
+// chunk, each takes g = floor(64 / c) lanes that split every tile (lane s of a group sweeps
+// slots s, s + g, ...), then a group reduction closes the particle: small leaves keep the
+// whole wavefront busy.  A list entry is staged only if its leaf box lies within the support
+// R h_i of some particle of the chunk (pointNodeGap2): the reference's list reaches R hmax
+// around the whole leaf box, most of which no particle's own support touches, and a leaf the
+// test drops holds no pair with q < R, so the pair sums are unchanged (for g = 1 even their
+// order is: the list order).
 template<bool PER>
-__global__ __launch_bounds__(kDensThreads) void gradientKernel(GradArgs A)
+__global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4))) void gradientKernel(GradArgs A)
 {
     __shared__ int    segStart[kDensWaves][kWave];
     __shared__ int    segJ[kDensWaves][kWave];
     __shared__ double tj[kDensWaves][9][kWave];
+    __shared__ double cp[kDensWaves][4][kWave];   // the chunk's particles: x, y, z, (R h)^2 slack
     const int w = threadIdx.x >> 6, lane = laneId();
     const int a = blockIdx.x * kDensWaves + w;
     if (a >= A.nLeaves) return;
     const int iLeaf = a;
     const uint32_t p0 = A.layout[iLeaf], p1 = A.layout[iLeaf + 1];
     const uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
-    DBox box{};
-    if constexpr (PER) box = A.boxes[A.leafSet[iLeaf]];
+    const DBox box = A.boxes[A.leafSet[iLeaf]];
     const double pmass = A.pmass;
     for (uint32_t q0 = p0; q0 < p1; q0 += kWave)
     {
-        const uint32_t i = q0 + lane;
-        const bool valid = i < p1;
-        const double xi = valid ? A.x[i] : 0.0, yi = valid ? A.y[i] : 0.0, zi = valid ? A.z[i] : 0.0;
-        const double vxi = valid ? A.vx[i] : 0.0, vyi = valid ? A.vy[i] : 0.0, vzi = valid ? A.vz[i] : 0.0;
-        const double axi = valid ? A.ax[i] : 0.0, ayi = valid ? A.ay[i] : 0.0, azi = valid ? A.az[i] : 0.0;
-        const double hi_ = valid ? A.h[i] : 1.0;
+        const int c = __builtin_amdgcn_readfirstlane((int)min((uint32_t)kWave, p1 - q0));
+        const int g = __builtin_amdgcn_readfirstlane(kWave / c);
+        const int grp = lane / g, sub = lane - grp * g;
+        const bool has = grp < c;
+        const uint32_t i = q0 + (uint32_t)(has ? grp : 0);
+        const double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+        const double vxi = A.vx[i], vyi = A.vy[i], vzi = A.vz[i];
+        const double axi = A.ax[i], ayi = A.ay[i], azi = A.az[i];
+        const double hi_ = A.h[i];
         const double hsi = 1.0 / (hi_ * hi_);
+        if (lane < c)
+        {
+            const uint32_t ic = q0 + (uint32_t)lane;
+            const double hc = A.h[ic];
+            cp[w][0][lane] = A.x[ic]; cp[w][1][lane] = A.y[ic]; cp[w][2][lane] = A.z[ic];
+            cp[w][3][lane] = (kRad * hc) * (kRad * hc) * kCullSlack;
+        }
+        waveLdsSync();
         double rhoi = 0, gradhi = 0, divs = 0, dv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, da[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         double rxx = 0, rxy = 0, rxz = 0, ryy = 0, ryz = 0, rzz = 0;
+        auto keep = [&](int jl) {
+            const int node = A.leafNode[jl];
+            for (int k = 0; k < c; ++k)
+                if (pointNodeGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.cx, A.cy, A.cz, A.sx, A.sy, A.sz, node,
+                                       box) <= cp[w][3][k])
+                    return true;
+            return false;
+        };
         auto load = [&](int l, int j) {
             tj[w][0][l] = A.x[j];  tj[w][1][l] = A.y[j];  tj[w][2][l] = A.z[j];
             tj[w][3][l] = A.vx[j]; tj[w][4][l] = A.vy[j]; tj[w][5][l] = A.vz[j];
             tj[w][6][l] = A.ax[j]; tj[w][7][l] = A.ay[j]; tj[w][8][l] = A.az[j];
         };
-        auto consume = [&](int nt) {
-            for (int k = 0; k < nt; ++k)
+        auto pair = [&](int k) {
+            double dx = xi - tj[w][0][k], dy = yi - tj[w][1][k], dz = zi - tj[w][2][k];
+            if constexpr (PER)
             {
-                double dx = xi - tj[w][0][k], dy = yi - tj[w][1][k], dz = zi - tj[w][2][k];
-                if constexpr (PER)
-                {
-                    if (fabs(dx) > 0.5 * box.l[0]) dx = dx - box.l[0] * copysign(1.0, dx);
-                    if (fabs(dy) > 0.5 * box.l[1]) dy = dy - box.l[1] * copysign(1.0, dy);
-                    if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
-                }
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                const double q2 = r2 * hsi;
-                if (q2 >= kRad2) continue;
-                const double q = sqrt(q2);
-                double wk, dwk;
-                kernWdW<double>(q2, q, wk, dwk);
-                rhoi += wk;
-                gradhi += -q * dwk - 3.0 * wk;
-                const double rij = sqrt(r2), rij1 = 1.0 / (rij + 2.220446049250313e-16);
-                const double rg = rij1 * dwk;
-                const double rx = dx * rg * pmass, ry = dy * rg * pmass, rz = dz * rg * pmass;
-                const double dvx = vxi - tj[w][3][k], dvy = vyi - tj[w][4][k], dvz = vzi - tj[w][5][k];
-                divs += dvx * rx + dvy * ry + dvz * rz;
-                dv[0] += dvx * rx; dv[1] += dvx * ry; dv[2] += dvx * rz;
-                dv[3] += dvy * rx; dv[4] += dvy * ry; dv[5] += dvy * rz;
-                dv[6] += dvz * rx; dv[7] += dvz * ry; dv[8] += dvz * rz;
-                const double dax = axi - tj[w][6][k], day = ayi - tj[w][7][k], daz = azi - tj[w][8][k];
-                da[0] += dax * rx; da[1] += dax * ry; da[2] += dax * rz;
-                da[3] += day * rx; da[4] += day * ry; da[5] += day * rz;
-                da[6] += daz * rx; da[7] += daz * ry; da[8] += daz * rz;
-                rxx -= dx * rx; rxy -= dx * ry; rxz -= dx * rz;
-                ryy -= dy * ry; ryz -= dy * rz; rzz -= dz * rz;
+                if (fabs(dx) > 0.5 * box.l[0]) dx = dx - box.l[0] * copysign(1.0, dx);
+                if (fabs(dy) > 0.5 * box.l[1]) dy = dy - box.l[1] * copysign(1.0, dy);
+                if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
             }
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double q2 = r2 * hsi;
+            if (q2 >= kRad2) return;
+            const double q = sqrt(q2);
+            double wk, dwk;
+            kernWdW<double>(q2, q, wk, dwk);
+            rhoi += wk;
+            gradhi += -q * dwk - 3.0 * wk;
+            const double rij = sqrt(r2), rij1 = 1.0 / (rij + 2.220446049250313e-16);
+            const double rg = rij1 * dwk;
+            const double rx = dx * rg * pmass, ry = dy * rg * pmass, rz = dz * rg * pmass;
+            const double dvx = vxi - tj[w][3][k], dvy = vyi - tj[w][4][k], dvz = vzi - tj[w][5][k];
+            divs += dvx * rx + dvy * ry + dvz * rz;
+            dv[0] += dvx * rx; dv[1] += dvx * ry; dv[2] += dvx * rz;
+            dv[3] += dvy * rx; dv[4] += dvy * ry; dv[5] += dvy * rz;
+            dv[6] += dvz * rx; dv[7] += dvz * ry; dv[8] += dvz * rz;
+            const double dax = axi - tj[w][6][k], day = ayi - tj[w][7][k], daz = azi - tj[w][8][k];
+            da[0] += dax * rx; da[1] += dax * ry; da[2] += dax * rz;
+            da[3] += day * rx; da[4] += day * ry; da[5] += day * rz;
+            da[6] += daz * rx; da[7] += daz * ry; da[8] += daz * rz;
+            rxx -= dx * rx; rxy -= dx * ry; rxz -= dx * rz;
+            ryy -= dy * ry; ryz -= dy * rz; rzz -= dz * rz;
         };
-        forEachTile(segStart[w], segJ[w], A.layout, A.jList, lo, hi, load, consume);
-        if (!valid) continue;
+        // lane sub of a group takes slots sub, sub + g, ... below nt (g = 1: slots 0 .. nt - 1,
+        // the list order, as the reference)
+        auto consume = [&](int nt) {
+            for (int k = sub; k < nt; k += g) pair(k);
+        };
+        forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, [](int) {}, consume);
+        // group sums: a tree over lanes sub .. g - 1 of each group (lane sub = 0 ends with them)
+        for (int off = 1; off < g; off <<= 1)
+        {
+            auto add = [&](double& v) {
+                const double o = __shfl_down(v, off, kWave);
+                if (sub + off < g) v += o;
+            };
+            add(rhoi); add(gradhi); add(divs);
+#pragma unroll
+            for (int f = 0; f < 9; ++f) { add(dv[f]); add(da[f]); }
+            add(rxx); add(rxy); add(rxz); add(ryy); add(ryz); add(rzz);
+        }
+        if (!has || sub != 0) continue;
 
         const double hi1 = 1.0 / hi_, hi31 = hi1 * hi1 * hi1, hi41 = hi31 * hi1;
         const double rho_i = rhoi * kCnormk * pmass * hi31;
@@ -164,7 +205,7 @@ __global__ __launch_bounds__(kDensThreads) void gradientKernel(GradArgs A)
         const double rm0 = ryy * rzz - ryz * ryz, rm1 = rxz * ryz - rzz * rxy, rm2 = rxy * ryz - rxz * ryy;
         const double rm3 = rzz * rxx - rxz * rxz, rm4 = rxy * rxz - rxx * ryz, rm5 = rxx * ryy - rxy * rxy;
         A.divv[i] = -divs * term;
-        double g[9], div_a;
+        double gm[9], div_a;
         if (fabs(denom) > 2.2250738585072014e-308)
         {
             const double dd = 1.0 / denom;
@@ -172,9 +213,9 @@ __global__ __launch_bounds__(kDensThreads) void gradientKernel(GradArgs A)
             for (int row = 0; row < 3; ++row)
             {
                 const double sx = dv[3 * row], sy = dv[3 * row + 1], sz = dv[3 * row + 2];
-                g[3 * row]     = -(sx * rm0 + sy * rm1 + sz * rm2) * dd;
-                g[3 * row + 1] = -(sx * rm1 + sy * rm3 + sz * rm4) * dd;
-                g[3 * row + 2] = -(sx * rm2 + sy * rm4 + sz * rm5) * dd;
+                gm[3 * row]     = -(sx * rm0 + sy * rm1 + sz * rm2) * dd;
+                gm[3 * row + 1] = -(sx * rm1 + sy * rm3 + sz * rm4) * dd;
+                gm[3 * row + 2] = -(sx * rm2 + sy * rm4 + sz * rm5) * dd;
             }
             double gd = 0.0;
 #pragma unroll
@@ -191,11 +232,12 @@ __global__ __launch_bounds__(kDensThreads) void gradientKernel(GradArgs A)
         else
         {
 #pragma unroll
-            for (int c = 0; c < 9; ++c) g[c] = -dv[c] * term;
+            for (int f = 0; f < 9; ++f) gm[f] = -dv[f] * term;
             div_a = -term * (da[0] + da[4] + da[8]);
         }
-        A.xi[i] = xiLimiter(g);
-        A.ddivvdt[i] = div_a - (g[0] * g[0] + g[4] * g[4] + g[8] * g[8] + 2.0 * (g[1] * g[3] + g[2] * g[6] + g[5] * g[7]));
+        A.xi[i] = xiLimiter(gm);
+        A.ddivvdt[i] =
+            div_a - (gm[0] * gm[0] + gm[4] * gm[4] + gm[8] * gm[8] + 2.0 * (gm[1] * gm[3] + gm[2] * gm[6] + gm[5] * gm[7]));
     }
 }
 

@@ -96,6 +96,9 @@ struct GradArgs
     double*         rho; double* gradh; double* divv; double* xi; double* ddivvdt;
     const DBox*     boxes;
     double          pmass;
+    const int*      leafNode;                                 // per-particle culling of list entries
+    const double*   cx; const double* cy; const double* cz;
+    const double*   sx; const double* sy; const double* sz;
 };
 
 struct ForceArgs
@@ -116,6 +119,10 @@ struct ForceArgs
     double          pmass, beta;
     int             pdv, shock;
     unsigned long long* counters;
+    const int*      leafNode;                                 // per-particle culling of list entries
+    const double*   cx; const double* cy; const double* cz;
+    const double*   sx; const double* sy; const double* sz;
+    const double*   hmaxLeaf;                                 // reach of a j-leaf's own supports
 };
 
 // mode 0: NR on the particles in state != 0 (FLAT_PARTICLE; state 2 = closure only);

@@ -343,6 +343,9 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         g.divv = s.divv.as<double>(); g.xi = s.xi.as<double>(); g.ddivvdt = s.ddivvdt.as<double>();
         g.boxes = s.box.as<DBox>();
         g.pmass = pmass;
+        g.leafNode = s.leafNode.as<int>();
+        g.cx = s.nodeCx.as<double>(); g.cy = s.nodeCy.as<double>(); g.cz = s.nodeCz.as<double>();
+        g.sx = s.nodeSx.as<double>(); g.sy = s.nodeSy.as<double>(); g.sz = s.nodeSz.as<double>();
         launchGradients(g, per, st);
         launchDeadGradients(na, n, s.divv.as<double>(), s.xi.as<double>(), s.ddivvdt.as<double>(), st);
         g2.rec(st);
@@ -411,6 +414,10 @@ ForceTimings computeForces(State& s, double pmass, double beta, double alphau, b
     a.pdv = pdv ? 1 : 0;
     a.shock = shock ? 1 : 0;
     a.counters = ctr;
+    a.leafNode = s.leafNode.as<int>();
+    a.cx = s.nodeCx.as<double>(); a.cy = s.nodeCy.as<double>(); a.cz = s.nodeCz.as<double>();
+    a.sx = s.nodeSx.as<double>(); a.sy = s.nodeSy.as<double>(); a.sz = s.nodeSz.as<double>();
+    a.hmaxLeaf = s.hmaxLeaf.as<double>();
     launchForce(a, per, disc, pdv && shock, st);
     launchDeadForce(na, n, s.fx.as<double>(), s.fy.as<double>(), s.fz.as<double>(), s.f4.as<double>(),
                     s.vsig.as<double>(), s.divvF.as<double>(), st);
