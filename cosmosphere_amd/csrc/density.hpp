@@ -123,6 +123,8 @@ struct ForceArgs
     const double*   cx; const double* cy; const double* cz;
     const double*   sx; const double* sy; const double* sz;
     const double*   hmaxLeaf;                                 // reach of a j-leaf's own supports
+    const float*    f32;      // mixed form: FP32 copies of hsqinv, hinv, grkfac, rhoh, rho1, pro2, cs,
+    size_t          f32Stride; // alpha, pres, auterm (array k at f32 + k f32Stride)
 };
 
 // mode 0: NR on the particles in state != 0 (FLAT_PARTICLE; state 2 = closure only);
@@ -157,7 +159,9 @@ int  counterStride();
 void launchForcePrep(const double* h, const double* gradh, const double* pro2, double* hsqinv, double* hinv,
                      double* rhoh, double* rho1, double* grkfac, double* pres, double* auterm, double* divfac, int n,
                      double pmass, double alphau, double hfact, hipStream_t st);
-void launchForce(const ForceArgs& a, bool periodic, bool disc, bool full, hipStream_t st);
+void launchForce(const ForceArgs& a, bool periodic, bool disc, bool full, bool mixed, hipStream_t st);
+// FP32 copies of the force factors for the mixed form (ForceArgs::f32 layout), n particles.
+void launchForceF32(const ForceArgs& a, int n, float* dst, hipStream_t st);
 void launchDeadForce(int first, int n, double* fx, double* fy, double* fz, double* f4, double* vsig, double* divv,
                      hipStream_t st);
 

@@ -4,9 +4,8 @@
  *
  * Restates src/force.cu:92-137 (forcePrepKernel) and :145-383 (sphForceKernelJList) for
  * gfx950: one wavefront per i-leaf over its symmetric list.  Only x, y, z and h of the
- * j-particles go through the LDS tile, because the cheap reject (r^2 >= R^2 max(hi^2,hj^2))
- * removes most candidates (~95% in the reference's measurements, force.cu:242-244); the
- * remaining j-fields are read only for a pair inside either support.
+ * j-particles' fields go through an LDS tile: each is staged once per list entry with
+ * coalesced loads and read by every particle of the chunk that evaluates it.
  */
 #include "density.hpp"
 #include "tile.hpp"
@@ -60,9 +59,11 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
 {
     __shared__ int    segStart[kForceWaves][kWave];
     __shared__ int    segJ[kForceWaves][kWave];
-    __shared__ double tx[kForceWaves][kWave], ty[kForceWaves][kWave], tz[kForceWaves][kWave];
-    __shared__ double th[kForceWaves][kWave];
-    __shared__ int    tj[kForceWaves][kWave];
+    // the tile: every j-field a pair reads, staged once per tile slot with coalesced loads
+    // (a j is accepted by several particles of the chunk: reading its fields per pair would
+    // be one scattered load per field per pair)
+    enum { X, Y, Z, H, VX, VY, VZ, HS, HI, GK, RH, R1, P2, CS, AL, U, PR, AU, NF };
+    __shared__ double tf[kForceWaves][NF][kWave];
     __shared__ double cp[kForceWaves][4][kWave];   // the chunk's particles: x, y, z, R h
     const int w = threadIdx.x >> 6, lane = laneId();
     const int a = blockIdx.x * kForceWaves + w;
@@ -110,10 +111,15 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
                 return false;
             };
             auto load = [&](int l, int j) {
-                tx[w][l] = A.x[j]; ty[w][l] = A.y[j]; tz[w][l] = A.z[j]; th[w][l] = A.h[j]; tj[w][l] = j;
+                tf[w][X][l] = A.x[j]; tf[w][Y][l] = A.y[j]; tf[w][Z][l] = A.z[j]; tf[w][H][l] = A.h[j];
+                tf[w][VX][l] = A.vx[j]; tf[w][VY][l] = A.vy[j]; tf[w][VZ][l] = A.vz[j];
+                tf[w][HS][l] = A.hsqinv[j]; tf[w][HI][l] = A.hinv[j]; tf[w][GK][l] = A.grkfac[j];
+                tf[w][RH][l] = A.rhoh[j]; tf[w][R1][l] = A.rho1[j]; tf[w][P2][l] = A.pro2[j];
+                tf[w][CS][l] = A.cs[j]; tf[w][AL][l] = A.alpha[j]; tf[w][U][l] = A.u[j];
+                tf[w][PR][l] = A.pres[j]; tf[w][AU][l] = A.auterm[j];
             };
             auto pair = [&](int k) {
-                double dx = xi - tx[w][k], dy = yi - ty[w][k], dz = zi - tz[w][k];
+                double dx = xi - tf[w][X][k], dy = yi - tf[w][Y][k], dz = zi - tf[w][Z][k];
                 if constexpr (PER)
                 {
                     if (fabs(dx) > 0.5 * box.l[0]) dx = dx - box.l[0] * copysign(1.0, dx);
@@ -121,17 +127,17 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
                     if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
                 }
                 const double dr2 = dx * dx + dy * dy + dz * dz;
-                const double hj = th[w][k];
+                const double hj = tf[w][H][k];
                 const double rmax2 = kRad2 * fmax(hi_ * hi_, hj * hj) * (1.0 + 1e-10);
                 if (!((dr2 > 0.0) && (dr2 < rmax2))) return;
-                const int j = tj[w][k];
-                const double qi2 = dr2 * hsi, qj2 = dr2 * A.hsqinv[j];
+                const double qi2 = dr2 * hsi, qj2 = dr2 * tf[w][HS][k];
                 const double dr = sqrt(dr2), rij1 = 1.0 / dr;
                 const double rx = dx * rij1, ry = dy * rij1, rz = dz * rij1;
-                const double dvx = vxi - A.vx[j], dvy = vyi - A.vy[j], dvz = vzi - A.vz[j];
+                const double dvx = vxi - tf[w][VX][k], dvy = vyi - tf[w][VY][k], dvz = vzi - tf[w][VZ][k];
                 const double projv = dvx * rx + dvy * ry + dvz * rz;
-                const double rhoj = A.rhoh[j], rho1j = A.rho1[j], pro2j = A.pro2[j];
-                const double vwj = A.cs[j], alj = A.alpha[j], enj = A.u[j], prj = A.pres[j], auj = A.auterm[j];
+                const double rhoj = tf[w][RH][k], rho1j = tf[w][R1][k], pro2j = tf[w][P2][k];
+                const double vwj = tf[w][CS][k], alj = tf[w][AL][k], enj = tf[w][U][k], prj = tf[w][PR][k];
+                const double auj = tf[w][AU][k];
                 const double vsigi = fmax(vwi - beta * projv, 0.0);
                 const double vsigavi = fmax(ali * vwi - beta * projv, 0.0);
                 const double vsigj = fmax(vwj - beta * projv, 0.0);
@@ -188,9 +194,9 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
                     counted = true;
                     vmax = fmax(vmax, pvmax);
                     double wk, dwk;
-                    const double q = dr * A.hinv[j];
+                    const double q = dr * tf[w][HI][k];
                     kernWdW<double>(qj2, q, wk, dwk);
-                    const double gk = dwk * A.grkfac[j];
+                    const double gk = dwk * tf[w][GK][k];
                     const double gp = pmass * (pro2j + qr2j) * gk;
                     jx -= gp * rx; jy -= gp * ry; jz -= gp * rz;
                     f4s += vsigu * denij * auj * gk;
@@ -236,6 +242,222 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
     }
 }
 
+// Mixed form of forceKernel: the same lists, culling, lanes and pair terms, with the pair
+// algebra in FP32.  Positions, velocities and u enter as FP64 differences rounded once to FP32
+// (a difference of two FP32-rounded values could lose all its digits); r^2, the support tests,
+// the square roots and reciprocals, the signal speeds, the viscosity and both kernel gradients
+// are FP32, from FP32 copies of the per-particle factors; each pair's terms go into FP32
+// partials that are added into FP64 accumulators once per tile (at most 64 pairs per lane).
+template<bool PER, bool DISC, bool FULL>
+__global__ __launch_bounds__(kForceThreads) void forceKernelMixed(ForceArgs A)
+{
+    __shared__ int    segStart[kForceWaves][kWave];
+    __shared__ int    segJ[kForceWaves][kWave];
+    // the tile: FP64 what enters a difference, FP32 the factors, all staged once per slot
+    enum { X, Y, Z, VX, VY, VZ, U, ND };
+    enum { H, HS, HI, GK, RH, R1, P2, CS, AL, PR, AU, NS };
+    __shared__ double td[kForceWaves][ND][kWave];
+    __shared__ float  ts[kForceWaves][NS][kWave];
+    __shared__ double cp[kForceWaves][4][kWave];   // the chunk's particles: x, y, z, R h
+    const int w = threadIdx.x >> 6, lane = laneId();
+    const int a = blockIdx.x * kForceWaves + w;
+    const size_t S = A.f32Stride;
+    auto F = [&](int k, uint32_t idx) { return A.f32[(size_t)k * S + idx]; };
+    unsigned long long nAcc = 0, nCand = 0;
+    if (a < A.nLeaves)
+    {
+        const int iLeaf = a;
+        const uint32_t p0 = A.layout[iLeaf], p1 = A.layout[iLeaf + 1];
+        const uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
+        const DBox box = A.boxes[A.leafSet[iLeaf]];
+        const float pm = (float)A.pmass, beta = (float)A.beta;
+        const float R2 = (float)kRad2;
+        for (uint32_t q0 = p0; q0 < p1; q0 += kWave)
+        {
+            const int c = __builtin_amdgcn_readfirstlane((int)min((uint32_t)kWave, p1 - q0));
+            const int g = __builtin_amdgcn_readfirstlane(kWave / c);
+            const int grp = lane / g, sub = lane - grp * g;
+            const bool has = grp < c;
+            const uint32_t i = q0 + (uint32_t)(has ? grp : 0);
+            const double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+            const double vxi = A.vx[i], vyi = A.vy[i], vzi = A.vz[i], eni = A.u[i];
+            const float hif = (float)A.h[i];
+            const float hsi = F(0, i), hinvi = F(1, i), gki = F(2, i), rhoi = F(3, i), rho1i = F(4, i);
+            const float pro2i = F(5, i), vwi = F(6, i), ali = F(7, i), pri = F(8, i), aui = F(9, i);
+            if (lane < c)
+            {
+                const uint32_t ic = q0 + (uint32_t)lane;
+                cp[w][0][lane] = A.x[ic]; cp[w][1][lane] = A.y[ic]; cp[w][2][lane] = A.z[ic];
+                cp[w][3][lane] = kRad * A.h[ic];
+            }
+            waveLdsSync();
+            double ix = 0, iy = 0, iz = 0, jx = 0, jy = 0, jz = 0, f4s = 0, divs = 0;
+            float pix = 0, piy = 0, piz = 0, pjx = 0, pjy = 0, pjz = 0, pf4 = 0, pdiv = 0, vmax = 0;
+            int acc = 0;
+            uint32_t cand = 0;
+            auto keep = [&](int jl) {
+                const int node = A.leafNode[jl];
+                double hj = A.hmaxLeaf[jl];
+                const double rj = (isfinite(hj) && hj > 0.0) ? kRad * hj : 0.0;
+                for (int k = 0; k < c; ++k)
+                {
+                    const double r = fmax(cp[w][3][k], rj);
+                    if (pointNodeGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.cx, A.cy, A.cz, A.sx, A.sy, A.sz,
+                                           node, box) <= r * r * kCullSlack)
+                        return true;
+                }
+                return false;
+            };
+            auto load = [&](int l, int j) {
+                td[w][X][l] = A.x[j]; td[w][Y][l] = A.y[j]; td[w][Z][l] = A.z[j];
+                td[w][VX][l] = A.vx[j]; td[w][VY][l] = A.vy[j]; td[w][VZ][l] = A.vz[j]; td[w][U][l] = A.u[j];
+                ts[w][H][l] = (float)A.h[j];
+                ts[w][HS][l] = F(0, j); ts[w][HI][l] = F(1, j); ts[w][GK][l] = F(2, j); ts[w][RH][l] = F(3, j);
+                ts[w][R1][l] = F(4, j); ts[w][P2][l] = F(5, j); ts[w][CS][l] = F(6, j); ts[w][AL][l] = F(7, j);
+                ts[w][PR][l] = F(8, j); ts[w][AU][l] = F(9, j);
+            };
+            auto pair = [&](int k) {
+                double dxd = xi - td[w][X][k], dyd = yi - td[w][Y][k], dzd = zi - td[w][Z][k];
+                if constexpr (PER)
+                {
+                    if (fabs(dxd) > 0.5 * box.l[0]) dxd = dxd - box.l[0] * copysign(1.0, dxd);
+                    if (fabs(dyd) > 0.5 * box.l[1]) dyd = dyd - box.l[1] * copysign(1.0, dyd);
+                    if (fabs(dzd) > 0.5 * box.l[2]) dzd = dzd - box.l[2] * copysign(1.0, dzd);
+                }
+                const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+                const float dr2 = dx * dx + dy * dy + dz * dz;
+                const float hj = ts[w][H][k];
+                const float rmax2 = R2 * fmaxf(hif * hif, hj * hj) * (1.0f + 1e-6f);
+                if (!((dr2 > 0.0f) && (dr2 < rmax2))) return;
+                const float qi2 = dr2 * hsi, qj2 = dr2 * ts[w][HS][k];
+                const float dr = __builtin_amdgcn_sqrtf(dr2), rij1 = __builtin_amdgcn_rcpf(dr);
+                const float rx = dx * rij1, ry = dy * rij1, rz = dz * rij1;
+                const float dvx = (float)(vxi - td[w][VX][k]), dvy = (float)(vyi - td[w][VY][k]);
+                const float dvz = (float)(vzi - td[w][VZ][k]);
+                const float projv = dvx * rx + dvy * ry + dvz * rz;
+                const float rhoj = ts[w][RH][k], rho1j = ts[w][R1][k], pro2j = ts[w][P2][k];
+                const float vwj = ts[w][CS][k], alj = ts[w][AL][k], prj = ts[w][PR][k], auj = ts[w][AU][k];
+                const float vsigi = fmaxf(vwi - beta * projv, 0.0f);
+                const float vsigavi = fmaxf(ali * vwi - beta * projv, 0.0f);
+                const float vsigj = fmaxf(vwj - beta * projv, 0.0f);
+                const float vsigavj = fmaxf(alj * vwj - beta * projv, 0.0f);
+                const float pvmax = fmaxf(vsigi, vsigj);
+                float qr2i = 0.0f, qr2j = 0.0f;
+                const float denij = (float)(eni - td[w][U][k]);
+                const float vsigu = __builtin_amdgcn_sqrtf(fabsf(pri - prj) * (2.0f * __builtin_amdgcn_rcpf(rhoi + rhoj)));
+                if constexpr (DISC)
+                {
+                    const float hori = hif * rij1, horj = hj * rij1;
+                    if (projv < 0.0f)
+                    {
+                        qr2i = -0.5f * rho1i * (ali * vwi - beta * projv) * hori * projv;
+                        qr2j = -0.5f * rho1j * (alj * vwj - beta * projv) * horj * projv;
+                    }
+                    else
+                    {
+                        qr2i = -0.5f * rho1i * ali * vwi * hori * projv;
+                        qr2j = -0.5f * rho1j * alj * vwj * horj * projv;
+                    }
+                }
+                else if (projv < 0.0f)
+                {
+                    qr2i = -0.5f * rho1i * vsigavi * projv;
+                    qr2j = -0.5f * rho1j * vsigavj * projv;
+                }
+                bool counted = false;
+                if (qi2 < R2)
+                {
+                    counted = true;
+                    vmax = fmaxf(vmax, pvmax);
+                    float wk, dwk;
+                    const float q = dr * hinvi;
+                    kernWdW<float>(qi2, q, wk, dwk);
+                    pdiv += pm * dwk * projv;
+                    const float gk = dwk * gki;
+                    const float gp = pm * (pro2i + qr2i) * gk;
+                    pix -= gp * rx; piy -= gp * ry; piz -= gp * rz;
+                    const float pdvt = pm * pro2i * projv * gk;
+                    const float diss = DISC ? -0.5f * pm * rho1i * ali * vwi * (hif * rij1) * projv * projv * gk
+                                            : pm * qr2i * projv * gk;
+                    const float dend = vsigu * denij * aui * gk;
+                    if constexpr (FULL) { pf4 += pdvt; pf4 += diss; }
+                    else
+                    {
+                        if (A.pdv) pf4 += pdvt;
+                        if (A.shock) pf4 += diss;
+                    }
+                    pf4 += dend;
+                }
+                if (qj2 < R2)
+                {
+                    counted = true;
+                    vmax = fmaxf(vmax, pvmax);
+                    float wk, dwk;
+                    const float q = dr * ts[w][HI][k];
+                    kernWdW<float>(qj2, q, wk, dwk);
+                    const float gk = dwk * ts[w][GK][k];
+                    const float gp = pm * (pro2j + qr2j) * gk;
+                    pjx -= gp * rx; pjy -= gp * ry; pjz -= gp * rz;
+                    pf4 += vsigu * denij * auj * gk;
+                }
+                acc += counted ? 1 : 0;
+            };
+            auto flush = [&]() {
+                ix += (double)pix; iy += (double)piy; iz += (double)piz;
+                jx += (double)pjx; jy += (double)pjy; jz += (double)pjz;
+                f4s += (double)pf4; divs += (double)pdiv;
+                pix = piy = piz = pjx = pjy = pjz = pf4 = pdiv = 0.0f;
+            };
+            auto consume = [&](int nt) {
+                if (sub == 0) cand += (uint32_t)nt;
+                for (int k = sub; k < nt; k += g) pair(k);
+                flush();
+            };
+            forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, [](int) {}, consume);
+            double vm = (double)vmax;
+            for (int off = 1; off < g; off <<= 1)
+            {
+                auto add = [&](double& v) {
+                    const double o = __shfl_down(v, off, kWave);
+                    if (sub + off < g) v += o;
+                };
+                add(ix); add(iy); add(iz); add(jx); add(jy); add(jz); add(f4s); add(divs);
+                const double om = __shfl_down(vm, off, kWave);
+                if (sub + off < g) vm = fmax(vm, om);
+                const int oa = __shfl_down(acc, off, kWave);
+                if (sub + off < g) acc += oa;
+            }
+            if (has && sub == 0)
+            {
+                A.fx[i] = ix + jx; A.fy[i] = iy + jy; A.fz[i] = iz + jz;
+                A.f4[i] = f4s; A.vsig[i] = vm;
+                A.divv[i] = -divs * A.divfac[i];
+                nAcc += (unsigned long long)acc;
+                nCand += cand;
+            }
+        }
+    }
+    nAcc = waveReduceSum(nAcc);
+    nCand = waveReduceSum(nCand);
+    if (laneId() == 0 && (nAcc | nCand))
+    {
+        unsigned long long* slot = A.counters + ((blockIdx.x * kForceWaves + w) % kCounterSlots) * kCounterStride;
+        atomicAdd(slot + 0, nAcc);
+        atomicAdd(slot + 1, nCand);
+    }
+}
+
+// FP32 copies of the force factors (ForceArgs::f32 layout).
+__global__ void forceF32Kernel(ForceArgs A, int n, float* __restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t S = A.f32Stride;
+    const double* src[10] = {A.hsqinv, A.hinv, A.grkfac, A.rhoh, A.rho1, A.pro2, A.cs, A.alpha, A.pres, A.auterm};
+#pragma unroll
+    for (int k = 0; k < 10; ++k) dst[(size_t)k * S + i] = (float)src[k][i];
+}
+
 __global__ void deadForceKernel(int first, int n, double* __restrict__ fx, double* __restrict__ fy,
                                 double* __restrict__ fz, double* __restrict__ f4, double* __restrict__ vsig,
                                 double* __restrict__ divv)
@@ -255,11 +477,22 @@ void launchForcePrep(const double* h, const double* gradh, const double* pro2, d
     LAUNCH_CHECK();
 }
 
-void launchForce(const ForceArgs& a, bool periodic, bool disc, bool full, hipStream_t st)
+void launchForceF32(const ForceArgs& a, int n, float* dst, hipStream_t st)
+{
+    if (n <= 0) return;
+    forceF32Kernel<<<iceil(n, 256), 256, 0, st>>>(a, n, dst);
+    LAUNCH_CHECK();
+}
+
+void launchForce(const ForceArgs& a, bool periodic, bool disc, bool full, bool mixed, hipStream_t st)
 {
     if (a.nLeaves <= 0) return;
     const int grid = iceil(a.nLeaves, kForceWaves);
-#define CSPH_F(P, D, F) forceKernel<P, D, F><<<grid, kForceThreads, 0, st>>>(a)
+#define CSPH_F(P, D, F)                                                             \
+    do {                                                                            \
+        if (mixed) forceKernelMixed<P, D, F><<<grid, kForceThreads, 0, st>>>(a);    \
+        else       forceKernel<P, D, F><<<grid, kForceThreads, 0, st>>>(a);         \
+    } while (0)
     if (periodic)
     {
         if (disc) { if (full) CSPH_F(true, true, true); else CSPH_F(true, true, false); }

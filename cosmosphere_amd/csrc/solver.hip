@@ -418,7 +418,15 @@ ForceTimings computeForces(State& s, double pmass, double beta, double alphau, b
     a.cx = s.nodeCx.as<double>(); a.cy = s.nodeCy.as<double>(); a.cz = s.nodeCz.as<double>();
     a.sx = s.nodeSx.as<double>(); a.sy = s.nodeSy.as<double>(); a.sz = s.nodeSz.as<double>();
     a.hmaxLeaf = s.hmaxLeaf.as<double>();
-    launchForce(a, per, disc, pdv && shock, st);
+    const bool mixed = s.precision == kMixed;
+    if (mixed)
+    {
+        a.f32Stride = (size_t)n + 1;
+        float* f32 = s.forceF32.get<float>(10 * a.f32Stride);
+        launchForceF32(a, n, f32, st);
+        a.f32 = f32;
+    }
+    launchForce(a, per, disc, pdv && shock, mixed, st);
     launchDeadForce(na, n, s.fx.as<double>(), s.fy.as<double>(), s.fz.as<double>(), s.f4.as<double>(),
                     s.vsig.as<double>(), s.divvF.as<double>(), st);
     e3.rec(st);

@@ -142,6 +142,7 @@ struct State
 
     // ---- force prep ----
     DevBuf hsqinv, hinv, rhoh, rho1, grkfac, pres, auterm, divfac;
+    DevBuf forceF32;          // mixed form: FP32 copies of the factors the pair terms read
 
     // ---- bookkeeping ----
     double   hfact = 0.0;

@@ -101,3 +101,64 @@ def test_force_repeat_reuses_lists():
     f2 = lib.download(FORCE_OUT, n)
     np.testing.assert_array_equal(f1, f2)
     np.testing.assert_array_equal(lib.download(HSML, n)[0], out["h"])
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(disc=1), dict(shock=0, quintic=False)])
+def test_force_mixed_vs_oracle(variant):
+    """Mixed form of the force pass (FP32 pair algebra from FP64 differences, FP64 sums) against
+    the FP64 oracle at the same h: accelerations, du/dt and div v within 2e-6 of their largest
+    magnitude, vsigmax within 2e-6 relative; momentum and energy conserved to FP32 rounding."""
+    quintic = variant.get("quintic", True)
+    c, v, u = setup(quintic=quintic)
+    n = c["n"]
+    disc, pdv, shock = variant.get("disc", 0), variant.get("pdv", 1), variant.get("shock", 1)
+    lib = Library(c["kernel"])
+    lib.set_option("precision", 1)
+    lib.set_option("max_iter", 100)
+    lib.set_option("tree_only", 0)
+    try:
+        out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], hfact=c["hfact"], v=v)
+        th = thermo(c, out["h"], u)
+        lib.upload_sorted(THERMO, th)
+        lib.force(n, c["pmass"], beta=2.0, alphau=1.0, disc_viscosity=disc, pdv_heating=pdv, shock_heating=shock)
+        f = lib.download(FORCE_OUT, n)
+        h_sorted = lib.array("h")
+        rho_sorted = lib.array("rho")
+        gradh_sorted = lib.array("gradh")
+    finally:
+        lib.set_option("precision", 0)
+    # the oracle at the GPU's own h and d(rho)/d(h), so only the force arithmetic differs
+    o = O.Oracle(quintic=c["quintic"])
+    o.set_particles(c["x"], c["y"], c["z"], c["h"], v=v, a=np.zeros((3, n)))
+    o.build()
+    assert o.density_solve(c["pmass"], 1e-4, c["hfact"], 100, 0, True) == 0
+    o.set_h_sorted(h_sorted)
+    o.closure(c["pmass"])
+    assert np.max(np.abs(o.get("rho") - rho_sorted) / rho_sorted) < 1e-6
+    o.set_thermo(*th)
+    o.force(c["pmass"], c["hfact"], beta=2.0, alphau=1.0, disc=disc, pdv=pdv, shock=shock)
+    order = o.get("order")
+
+    def host(name):
+        a = np.empty(n)
+        a[order] = o.get(name)
+        return a
+
+    fmax = np.abs(np.stack([host("fx"), host("fy"), host("fz")])).max()
+    errs = {}
+    for k, name in enumerate(("fx", "fy", "fz")):
+        errs[name] = np.max(np.abs(f[k] - host(name))) / fmax
+    du = host("du")
+    errs["du"] = np.max(np.abs(f[3] - du)) / np.abs(du).max()
+    vs = host("vsig")
+    errs["vsig"] = np.max(np.abs(f[4] - vs) / np.maximum(vs, 1e-300))
+    dv = host("divvF")
+    errs["divv"] = np.max(np.abs(f[5] - dv)) / np.abs(dv).max()
+    print("mixed force errors", errs, flush=True)
+    for k, e in errs.items():
+        assert e < 2e-6, (k, e)
+    m = c["pmass"]
+    assert abs(f[0].sum()) < 1e-6 * np.abs(f[0]).sum()
+    if pdv and shock and not disc:
+        power = v[0] * f[0] + v[1] * f[1] + v[2] * f[2] + f[3]
+        assert abs((m * power).sum()) < 1e-6 * np.abs(m * power).sum()
