@@ -10,9 +10,20 @@ lists, density kernel, compaction) until every particle has converged -- the ref
 solveDensH (src/density_base.cu:720-1015) in FLAT_PARTICLE mode, density only.  The mixed
 form also evaluates rho(h) at the converged h (its closure), so its outputs are consistent.
 
+Before the timed steps the mixed form is autotuned (untimed, --tune-budget seconds, default
+45): the library times complete solves of the same inputs over its speed options (loop driver,
+leaf bucket, item mapping, persistent grid, graph depth, queue order), keeps the fastest and puts
+the inputs back; the chosen options are in the JSON line.  The FP64 path keeps the library's
+defaults unless --tune-forms names it.
+
+Other workloads: --config C1 | C2 (single sets), C4 (ensemble of 256 sets solved as one batch,
+against 256 sequential plain FP64 solves), C5 (density + force on a 16M-particle Plummer
+mixture with velocities, a 10-step drift loop through the reference's call sequence).
+
 Prints ONE JSON line on stdout; progress goes to stderr.
 
-  python bench.py [--steps K] [--warmup W] [--config C3] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--config C3] [--no-cpu] [--tune-budget S]
+                  [--opt name=value ...]
 """
 import argparse
 import json
@@ -399,13 +410,17 @@ def main():
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--force-steps", type=int, default=None, help="C5: drift steps after the first solve (default 10)")
     ap.add_argument("--tune-budget", type=float, default=None,
-                    help="seconds of autotuning before the timed steps (default: 60 for C4, 0 otherwise)")
+                    help="seconds of autotuning per precision form before the timed steps (default: 60 for C4, "
+                         "45 for C1-C3, 0 for C5)")
+    ap.add_argument("--tune-forms", default="mixed",
+                    help="comma-separated precision forms of a single-set workload to autotune (default: mixed; the "
+                         "FP64 path keeps the library's default options)")
     ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
                     help="library option (cosmo_ext_set_option), e.g. loop=0 for the host-driven NR loop")
     args = ap.parse_args()
     args.opts = [(kv.split("=")[0], float(kv.split("=")[1])) for kv in args.opt]
     if args.tune_budget is None:
-        args.tune_budget = 60.0 if args.config == "C4" else 0.0
+        args.tune_budget = 60.0 if args.config == "C4" else (0.0 if args.config.startswith("C5") else 45.0)
     if args.gpus != 1:
         print("bench.py measures one GPU: this solver runs a particle set (or an ensemble of sets) on one MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -431,11 +446,29 @@ def main():
     lib.stash_inputs()
 
     res = {}
+    tuned = {}
     for name, prec in (("fp64", 0), ("mixed", 1)):
         if args.only not in ("both", name):
             continue
         log("precision %s" % name)
-        res[name] = run_precision(lib, c, prec, args.steps, args.warmup, args.opts)
+        if args.tune_budget > 0 and name in args.tune_forms.split(","):
+            # untimed: the autotuner times complete solves of these inputs and leaves the fastest
+            # options in force (the inputs are put back afterwards)
+            lib.set_option("precision", prec)
+            lib.set_option("max_iter", c.get("max_iter", 100))
+            lib.set_option("closure", -1)
+            lib.set_option("tree_only", 0)
+            for k, v in DEFAULT_ITEMS.items():
+                lib.set_option(k, v)
+            for k, v in args.opts:
+                lib.set_option(k, v)
+            lib.restore_inputs()
+            tuned[name] = lib.autotune(n, c["pmass"], tolh=1e-4, hfact=c["hfact"], budget_s=args.tune_budget)
+            log("  autotune %s: %s" % (name, tuned[name]))
+            args_opts = [(k, tuned[name][k]) for k in Library.TUNE_OPTIONS]
+        else:
+            args_opts = args.opts
+        res[name] = run_precision(lib, c, prec, args.steps, args.warmup, args_opts)
         r = res[name]
         log("  %s: %.1f ms/step, %.3e h-solves/s, %.3e pairs/s (kernel), kernel %.2f TFLOP/s, iters %.1f"
             % (name, r["ms_per_step"], r["h_solves_per_s"], r["pairs_per_s_kernel"], r["kernel_tflops"],
@@ -467,6 +500,8 @@ def main():
         "candidates_per_step": head["candidates_per_step"],
         "iterations": head["iterations"],
         "phase_ms": head["phase_ms"],
+        "options": {k: lib.stat(k) for k in Library.TUNE_OPTIONS},
+        "autotune": tuned,
         "roofline": {"bound": "valu", "achieved": head["kernel_tflops"], "peak": peak, "unit": "TFLOP/s",
                      "frac": head["kernel_tflops"] / peak, "traffic": traffic,
                      "traffic_source": traffic_src,

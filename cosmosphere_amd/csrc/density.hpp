@@ -26,7 +26,7 @@ struct PassCtl
     uint32_t perParticle;   // the current pass uses per-particle items for sparse leaves
     uint32_t isNR;          // the current pass holds NR evaluations (else closures only)
     uint32_t queueHead;     // next position of the work queue
-    uint32_t qBucket[32];   // cost-order buckets: counts, then running bases
+    uint32_t qBucket[32];   // work queue: [0] largest cost bucket, [1] heavy items at its head
     unsigned long long nIterating;      // particles in state 1 at the start of the pass
     unsigned long long listNeed;        // list entries of the current pass
     unsigned long long tBegin;          // wall clock at the start of the pass
@@ -66,7 +66,8 @@ struct DensArgs
     int             closure;    // converged particles pass through state 2
     unsigned long long* counters;
     PassCtl*        ctl;        // device-driven pass: persistent waves pull items from a queue
-    const int*      qOrder;     // ... in this order (cost-descending), nullptr: item order
+    const int*      qOrder;     // ... heavy items qOrder[0, ctl->qBucket[1]) first, then every item
+    const uint8_t*  qFlag;      //     not flagged heavy, in item order (nullptr: item order only)
 };
 
 // Device work counters (u64, summed over the counter slots).

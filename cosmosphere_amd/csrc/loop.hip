@@ -6,18 +6,18 @@
  * pass takes its counts from a small control block in device memory (PassCtl), so a pass is
  * a fixed sequence of launches: K passes are captured once in a HIP graph and the graph is
  * replayed until the loop is over, one host synchronisation per replay.  The density kernel
- * runs as a persistent grid whose wavefronts pull work items from the pass's queue, in
- * cost-descending order (longest lists first), so the pass does not end on a long item
- * started last.
+ * runs as a persistent grid whose wavefronts pull work items from the pass's queue: the
+ * items with the longest lists first, so the pass does not end on a long item started last,
+ * then the rest in the leaves' Hilbert order, so neighbouring items share j-particles in L2.
  *
  * One pass (the pass of src/density_base.cu:782-887):
  *   passBegin   halt (no active leaf, or the NR pass cap reached), NR or closure-only,
- *               item form; clears the queue, the cost buckets and the kernel span
+ *               item form; clears the queue, the heavy-item count and the kernel span
  *   counters    cleared (fill kernel: no memset node in the graph)
  *   hmax        hmax and hRem of the active leaves
  *   lists       count walk, scan, listCheck (capacity), fill walk: the tight gather lists
  *   items       scan of the per-leaf item counts, item fill, itemCount
- *   order       cost buckets of the items (log2 of the list length), bases, scatter
+ *   order       the heavy items (list length within 4x of the longest) to the queue's head
  *   density     both item forms as persistent grids; the one not selected exits at once
  *   compaction  leaf flags, scan, scatter, passEnd (log record, counts of the next pass),
  *               copy back
@@ -93,7 +93,7 @@ __global__ void itemCountKernel(PassCtl* __restrict__ C, const uint32_t* __restr
     C->nItems = C->perParticle ? (C->nAL ? itemOff[C->nAL] : 0u) : C->nAL;
 }
 
-// Cost bucket of an item: floor(log2) of its leaf's list length, the longest first.
+// Cost bucket of an item: floor(log2) of its leaf's list length.
 __device__ inline int costBucket(const PassCtl* C, const int* __restrict__ itemPos, const uint32_t* __restrict__ jCount,
                                  int k)
 {
@@ -103,59 +103,42 @@ __device__ inline int costBucket(const PassCtl* C, const int* __restrict__ itemP
 }
 
 constexpr int kCostThreads = 256;
+constexpr int kHeavySpan   = 2;   // heavy: a list at least 1/4 as long as the longest (in buckets)
+constexpr int kHeavyMean   = 8;   // ... and at least 8 times the mean list length of the pass
 
-__global__ __launch_bounds__(kCostThreads) void costCountKernel(PassCtl* __restrict__ C,
-                                                                 const int* __restrict__ itemPos,
-                                                                 const uint32_t* __restrict__ jCount)
+// Largest cost bucket of the pass's items -> qBucket[0].
+__global__ __launch_bounds__(kCostThreads) void costMaxKernel(PassCtl* __restrict__ C, const int* __restrict__ itemPos,
+                                                               const uint32_t* __restrict__ jCount)
 {
-    __shared__ uint32_t hist[kCostBuckets];
     if (C->skip) return;
     const int n = (int)C->nItems;
-    if (threadIdx.x < kCostBuckets) hist[threadIdx.x] = 0u;
-    __syncthreads();
+    int m = 0;
     for (int k = blockIdx.x * kCostThreads + threadIdx.x; k < n; k += gridDim.x * kCostThreads)
-        atomicAdd(&hist[costBucket(C, itemPos, jCount, k)], 1u);
-    __syncthreads();
-    if (threadIdx.x < kCostBuckets && hist[threadIdx.x]) atomicAdd(&C->qBucket[threadIdx.x], hist[threadIdx.x]);
+        m = max(m, costBucket(C, itemPos, jCount, k));
+    m = waveReduceMax(m);
+    if (laneId() == 0 && m > 0) atomicMax(&C->qBucket[0], (uint32_t)m);
 }
 
-// Bucket counts -> first queue position of each bucket, buckets in descending cost.
-__global__ void costBaseKernel(PassCtl* __restrict__ C)
+// The work queue: the heavy items first (qOrder[0, qBucket[1]), in no particular order: there
+// are few, and started last they would be the pass's tail), then every item in its own order
+// (the leaves' Hilbert order, which keeps the j-particles that neighbouring items share in L2),
+// skipping the heavy ones (qFlag).  A workload whose lists are all alike has no heavy item.
+__global__ __launch_bounds__(kCostThreads) void costHeavyKernel(PassCtl* __restrict__ C,
+                                                                 const int* __restrict__ itemPos,
+                                                                 const uint32_t* __restrict__ jCount,
+                                                                 int* __restrict__ qOrder, uint8_t* __restrict__ qFlag)
 {
-    if (C->skip || threadIdx.x != 0) return;
-    uint32_t base = 0u;
-    for (int b = kCostBuckets - 1; b >= 0; --b)
-    {
-        const uint32_t c = C->qBucket[b];
-        C->qBucket[b] = base;
-        base += c;
-    }
-}
-
-// Queue positions: a tile of 256 items reserves its range of each bucket with one atomic
-// per bucket, and its items take consecutive positions in it.
-__global__ __launch_bounds__(kCostThreads) void costScatterKernel(PassCtl* __restrict__ C,
-                                                                   const int* __restrict__ itemPos,
-                                                                   const uint32_t* __restrict__ jCount,
-                                                                   int* __restrict__ qOrder)
-{
-    __shared__ uint32_t hist[kCostBuckets], base[kCostBuckets];
     if (C->skip) return;
     const int n = (int)C->nItems;
-    for (int k0 = blockIdx.x * kCostThreads; k0 < n; k0 += gridDim.x * kCostThreads)
+    const int cut = (int)C->qBucket[0] - kHeavySpan;
+    const unsigned long long mean = C->nAL ? C->listNeed / C->nAL : 0ull;   // entries per active leaf
+    for (int k = blockIdx.x * kCostThreads + threadIdx.x; k < n; k += gridDim.x * kCostThreads)
     {
-        if (threadIdx.x < kCostBuckets) hist[threadIdx.x] = 0u;
-        __syncthreads();
-        const int k = k0 + (int)threadIdx.x;
-        const int b = k < n ? costBucket(C, itemPos, jCount, k) : -1;
-        uint32_t rank = 0u;
-        if (b >= 0) rank = atomicAdd(&hist[b], 1u);
-        __syncthreads();
-        if (threadIdx.x < kCostBuckets && hist[threadIdx.x])
-            base[threadIdx.x] = atomicAdd(&C->qBucket[threadIdx.x], hist[threadIdx.x]);
-        __syncthreads();
-        if (b >= 0) qOrder[base[b] + rank] = k;
-        __syncthreads();
+        const int a = C->perParticle ? itemPos[k] : k;
+        const bool heavy =
+            costBucket(C, itemPos, jCount, k) >= cut && (unsigned long long)jCount[a] >= kHeavyMean * mean;
+        qFlag[k] = heavy ? 1 : 0;
+        if (heavy) qOrder[atomicAdd(&C->qBucket[1], 1u)] = k;
     }
 }
 
@@ -239,6 +222,7 @@ struct Plan
     int*        itemPos;
     int*        itemPart;
     int*        qOrder;
+    uint8_t*    qFlag;
     uint32_t*   jc;
     uint32_t*   jo;
     int*        jList;
@@ -274,9 +258,8 @@ void enqueuePass(State& s, const Plan& P)
     LAUNCH_CHECK();
     if (P.lpt)
     {
-        costCountKernel<<<P.costBlocks, kCostThreads, 0, st>>>(P.ctl, P.itemPos, P.jc);
-        costBaseKernel<<<1, 1, 0, st>>>(P.ctl);
-        costScatterKernel<<<P.costBlocks, kCostThreads, 0, st>>>(P.ctl, P.itemPos, P.jc, P.qOrder);
+        costMaxKernel<<<P.costBlocks, kCostThreads, 0, st>>>(P.ctl, P.itemPos, P.jc);
+        costHeavyKernel<<<P.costBlocks, kCostThreads, 0, st>>>(P.ctl, P.itemPos, P.jc, P.qOrder, P.qFlag);
         LAUNCH_CHECK();
     }
     launchDensityPersistent(P.da, P.precision, P.periodic, P.blocks, st);
@@ -409,6 +392,7 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
     P.ctl = s.passCtl.get<PassCtl>(1);
     P.log = s.passLogDev.get<PassLogDev>((size_t)P.logCap);
     P.qOrder = s.qOrder.get<int>((size_t)std::max(na, L) + 1);
+    P.qFlag = s.qFlag.get<uint8_t>((size_t)std::max(na, L) + 1);
     P.jc = s.jCount.get<uint32_t>((size_t)L + 1);
     P.jo = s.jOffset.get<uint32_t>((size_t)L + 2);
     P.counters = s.counters.get<unsigned long long>((size_t)kCounterSlots * kCounterStride + 64);
@@ -445,6 +429,7 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
     da.hmaxLeaf = s.hRemLeaf.as<double>();   // FLAT passes: bounds every h evaluated
     da.ctl = P.ctl;
     da.qOrder = P.lpt ? P.qOrder : nullptr;
+    da.qFlag = P.lpt ? P.qFlag : nullptr;
     da.itemPos = P.itemPos;
     da.itemPart = P.itemPart;
     da.jOffset = P.jo;

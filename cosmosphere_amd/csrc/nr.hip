@@ -436,13 +436,21 @@ void densityKernel(DensArgs A)
         if (C->skip || C->perParticle != (PP ? 1u : 0u)) return;   // the whole block
         if (threadIdx.x == 0) atomicMin(&C->tStart, (unsigned long long)wall_clock64());
         const int nItems = (int)C->nItems;
+        const int nHeavy = A.qOrder ? (int)C->qBucket[1] : 0;
         for (;;)
         {
             int k = 0;
             if (lane == 0) k = (int)atomicAdd(&C->queueHead, 1u);
             k = __builtin_amdgcn_readfirstlane(k);
+            if (k < nHeavy)
+            {
+                run(A.qOrder[k], ctr);
+                continue;
+            }
+            k -= nHeavy;
             if (k >= nItems) break;
-            run(A.qOrder ? A.qOrder[k] : k, ctr);
+            if (nHeavy && A.qFlag[k]) continue;   // taken from the head already
+            run(k, ctr);
         }
     }
     else

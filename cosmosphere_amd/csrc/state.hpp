@@ -138,7 +138,7 @@ struct State
     DevBuf ovf;               // int[2] list truncation / stack drop (never expected)
     DevBuf passCtl;           // device-driven loop: control block, pass records, work queue order
     DevBuf passLogDev;
-    DevBuf qOrder;
+    DevBuf qOrder, qFlag;
 
     // ---- force prep ----
     DevBuf hsqinv, hinv, rhoh, rho1, grkfac, pres, auterm, divfac;
