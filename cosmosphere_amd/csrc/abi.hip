@@ -338,6 +338,7 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "graph") s.useGraph = (int)value;
     else if (nm == "graph_passes") s.graphPasses = std::max(1, (int)value);
     else if (nm == "lpt") s.lpt = (int)value;
+    else if (nm == "cull") s.cull = (int)value;
     else if (nm == "loop_blocks_scale") s.loopBlockScale = std::max(1, (int)value);
     else if (nm == "loop_list_cap") s.loopListCap = (long long)value;
     else
@@ -396,6 +397,7 @@ extern "C" double cosmo_ext_get_stat(const char* name)
     if (nm == "graph") return s.useGraph;
     if (nm == "graph_passes") return s.graphPasses;
     if (nm == "lpt") return s.lpt;
+    if (nm == "cull") return s.cull;
     if (nm == "mapping") return s.mapping;
     if (nm == "sparse") return s.sparse;
     if (nm == "tail_leaves") return s.tailLeaves;

@@ -64,6 +64,7 @@ struct DensArgs
     double          pmass, tolh, hfact;
     const double*   pmassSet;   // ensemble: particle mass of each set (nullptr: pmass)
     int             closure;    // converged particles pass through state 2
+    int             cull;       // whole-leaf items: stage only list entries some particle's support reaches
     unsigned long long* counters;
     PassCtl*        ctl;        // device-driven pass: persistent waves pull items from a queue
     const int*      qOrder;     // ... heavy items qOrder[0, ctl->qBucket[1]) first, then every item

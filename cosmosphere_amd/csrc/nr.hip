@@ -228,7 +228,9 @@ __device__ inline void particleSums(double s0, double s1, double& wsum, double& 
 // the whole-leaf path low when it does not).
 // QUEUE: the device-driven pass's persistent grid (A.ctl), a separate instance so the
 // register budget of the host-driven kernels is unchanged; held to 80 VGPRs (6 waves per SIMD).
-template<class Real, bool PER, int MODE, bool PP, bool QUEUE = false>
+// CULL: whole-leaf items stage only the list entries some particle's own support reaches (a
+// separate instance: the culling table and test cost registers the plain instances keep).
+template<class Real, bool PER, int MODE, bool PP, bool QUEUE = false, bool CULL = false>
 __global__ __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(QUEUE ? 6 : 1)))
 void densityKernel(DensArgs A)
 {
@@ -236,6 +238,7 @@ void densityKernel(DensArgs A)
     __shared__ int segStart[kNrWaves][kWave];
     __shared__ int segJ[kNrWaves][kWave];
     __shared__ int rankLane[kNrWaves][kWave];   // whole-leaf items: k-th particle to evaluate -> lane
+    __shared__ double cpw[CULL ? kNrWaves : 1][4][CULL ? kWave : 1];   // ... its x, y, z, (R h)^2 slack
     // whole-leaf tiles: mixed = fixed-point offsets (x, y, z, pad), one 16-byte LDS read per j;
     // plain = FP64 positions
     __shared__ int4   tq[kMixed ? kNrWaves : 1][kMixed ? 2 * kWave : 1];
@@ -329,8 +332,36 @@ void densityKernel(DensArgs A)
                 if (mask == 0) continue;
                 const int c = __builtin_amdgcn_readfirstlane(__popcll(mask));
                 const int g = __builtin_amdgcn_readfirstlane(kWave / c);   // lanes per particle
-                if (want) rankLane[w][__popcll(mask & lanemaskLt())] = lane;
+                if (want)
+                {
+                    const int r = __popcll(mask & lanemaskLt());
+                    rankLane[w][r] = lane;
+                    if constexpr (CULL)
+                    {
+                        const double hc = A.h[ic];
+                        cpw[w][0][r] = A.x[ic]; cpw[w][1][r] = A.y[ic]; cpw[w][2][r] = A.z[ic];
+                        cpw[w][3][r] = (kRad * hc) * (kRad * hc) * kCullSlack;
+                    }
+                }
                 waveLdsSync();
+                // with culling, a list entry is staged only if its leaf box lies within the support
+                // of one of the c particles (a dropped leaf holds no pair with q < R for them)
+                auto keep = [&](int jl) {
+                    if constexpr (CULL)
+                    {
+                        const int jn = A.leafNode[jl];
+                        for (int k = 0; k < c; ++k)
+                            if (pointNodeGap2<PER>(cpw[w][0][k], cpw[w][1][k], cpw[w][2][k], A.cx, A.cy, A.cz, A.sx,
+                                                   A.sy, A.sz, jn, box) <= cpw[w][3][k])
+                                return true;
+                        return false;
+                    }
+                    else
+                    {
+                        (void)jl;
+                        return true;
+                    }
+                };
                 const int grp = lane / g, sub = lane - grp * g;
                 const bool has = grp < c;
                 const uint32_t i = has ? q0 + (uint32_t)rankLane[w][grp] : p0;
@@ -407,8 +438,7 @@ void densityKernel(DensArgs A)
                     }
                     cand += sub == 0 ? (uint32_t)nt : 0u;
                 };
-                forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, [](int) { return true; }, load, pad,
-                              consume);
+                forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, pad, consume);
                 if (has)
                 {
                     const int slot = isClosureEval<MODE>(st) ? 2 : 0;
@@ -573,10 +603,15 @@ void launchDensity(const DensArgs& a, int precision, int mode, bool periodic, hi
     const int grid = iceil(a.nItems, kNrWaves);
     const bool pp = a.itemPart != nullptr;
 #define CSPH_DENS(R, P, M, Q) densityKernel<R, P, M, Q><<<grid, kNrThreads, 0, st>>>(a)
+#define CSPH_DENS_C(R, P, M, Q)                                                          \
+    do {                                                                                 \
+        if (a.cull) densityKernel<R, P, M, Q, false, true><<<grid, kNrThreads, 0, st>>>(a); \
+        else        CSPH_DENS(R, P, M, Q);                                               \
+    } while (0)
 #define CSPH_DENS_PP(R, P, M)                     \
     do {                                          \
-        if (pp) CSPH_DENS(R, P, M, true);         \
-        else    CSPH_DENS(R, P, M, false);        \
+        if (pp) CSPH_DENS_C(R, P, M, true);       \
+        else    CSPH_DENS_C(R, P, M, false);      \
     } while (0)
 #define CSPH_DENS_MODE(R, P)                      \
     do {                                          \
@@ -596,16 +631,25 @@ void launchDensity(const DensArgs& a, int precision, int mode, bool periodic, hi
     }
 #undef CSPH_DENS_MODE
 #undef CSPH_DENS_PP
+#undef CSPH_DENS_C
 #undef CSPH_DENS
     LAUNCH_CHECK();
 }
 
 void launchDensityPersistent(const DensArgs& a, int precision, bool periodic, int blocks, hipStream_t st)
 {
-#define CSPH_DENS_P(R, P)                                                   \
-    do {                                                                    \
-        densityKernel<R, P, 0, false, true><<<blocks, kNrThreads, 0, st>>>(a); \
-        densityKernel<R, P, 0, true, true><<<blocks, kNrThreads, 0, st>>>(a);  \
+#define CSPH_DENS_P(R, P)                                                             \
+    do {                                                                              \
+        if (a.cull)                                                                   \
+        {                                                                             \
+            densityKernel<R, P, 0, false, true, true><<<blocks, kNrThreads, 0, st>>>(a); \
+            densityKernel<R, P, 0, true, true, true><<<blocks, kNrThreads, 0, st>>>(a);  \
+        }                                                                             \
+        else                                                                          \
+        {                                                                             \
+            densityKernel<R, P, 0, false, true><<<blocks, kNrThreads, 0, st>>>(a);       \
+            densityKernel<R, P, 0, true, true><<<blocks, kNrThreads, 0, st>>>(a);        \
+        }                                                                             \
     } while (0)
     if (precision == kMixed)
     {

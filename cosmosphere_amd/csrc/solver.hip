@@ -127,6 +127,7 @@ static DensArgs densArgs(State& s, const int* leaves, int nLeaves, double pmass,
     a.pmassSet = s.usePmassSet ? s.pmassSet.as<double>() : nullptr;
     a.counters = counters;
     a.nItems = nLeaves;   // whole-leaf items unless the caller sets the item arrays
+    a.cull = s.cull;
     return a;
 }
 

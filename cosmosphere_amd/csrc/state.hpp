@@ -157,7 +157,8 @@ struct State
     int      loopMode = 1;         // NR passes: 0 host-driven, 1 device-driven (FLAT_PARTICLE)
     int      useGraph = 1;         // device-driven passes replayed as a captured HIP graph
     int      graphPasses = 4;      // passes per graph replay (per host round trip)
-    int      lpt = 1;              // work queue in cost-descending order
+    int      lpt = 1;              // work queue: the heaviest items first
+    int      cull = 1;             // NR whole-leaf items: per-particle culling of list entries
     int      loopBlockScale = 4;   // persistent density grid: resident blocks x scale / 4
     int      loopBlocks = 0, loopBlocksPrecision = -1, loopBlocksPeriodic = -1;
     long long loopListCap = 0;     // verification: starting list capacity of the device-driven loop (0: auto)

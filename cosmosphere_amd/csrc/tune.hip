@@ -12,7 +12,8 @@
  *   mapping       wave per leaf, wave per particle, or the automatic switch (sparse, tailLeaves)
  *   blockScale    persistent grid size (resident blocks, or oversubscribed)
  *   graphPasses   passes per graph replay (idle passes at the end vs host round trips)
- *   lpt           cost-ordered work queue
+ *   lpt           heaviest items first in the work queue
+ *   cull          per-particle culling of the NR whole-leaf items' list entries
  *
  * by a coordinate search: one option at a time, each candidate value timed by a solve (a second
  * solve confirms a candidate within 5% of the best), the best kept, sweeps repeated until
@@ -34,7 +35,8 @@ namespace csph
 
 TuneParams currentParams(const State& s)
 {
-    return TuneParams{s.loopMode, s.bucket, s.mapping, s.sparse, s.tailLeaves, s.loopBlockScale, s.graphPasses, s.lpt};
+    return TuneParams{s.loopMode, s.bucket, s.mapping, s.sparse, s.tailLeaves, s.loopBlockScale, s.graphPasses, s.lpt,
+                      s.cull};
 }
 
 void applyParams(State& s, const TuneParams& p)
@@ -47,23 +49,24 @@ void applyParams(State& s, const TuneParams& p)
     s.loopBlockScale = p.blockScale;
     s.graphPasses = p.graphPasses;
     s.lpt = p.lpt;
+    s.cull = p.cull;
 }
 
 std::string describe(const TuneParams& p)
 {
     char b[256];
     std::snprintf(b, sizeof(b), "loop=%d bucket=%d mapping=%d sparse=%d tail_leaves=%d loop_blocks_scale=%d "
-                                "graph_passes=%d lpt=%d",
-                  p.loop, p.bucket, p.mapping, p.sparse, p.tailLeaves, p.blockScale, p.graphPasses, p.lpt);
+                                "graph_passes=%d lpt=%d cull=%d",
+                  p.loop, p.bucket, p.mapping, p.sparse, p.tailLeaves, p.blockScale, p.graphPasses, p.lpt, p.cull);
     return b;
 }
 
 static bool parse(const std::string& text, TuneParams& p)
 {
     return std::sscanf(text.c_str(), "loop=%d bucket=%d mapping=%d sparse=%d tail_leaves=%d loop_blocks_scale=%d "
-                                     "graph_passes=%d lpt=%d",
+                                     "graph_passes=%d lpt=%d cull=%d",
                        &p.loop, &p.bucket, &p.mapping, &p.sparse, &p.tailLeaves, &p.blockScale, &p.graphPasses,
-                       &p.lpt) == 8;
+                       &p.lpt, &p.cull) == 9;
 }
 
 namespace
@@ -193,6 +196,7 @@ TuneResult autotune(State& s, const std::string& signature, const std::function<
         {&TuneParams::blockScale, {4, 8, 2}, deviceLoopOn},
         {&TuneParams::graphPasses, {4, 1, 2, 8}, deviceLoopOn},
         {&TuneParams::lpt, {1, 0}, deviceLoopOn},
+        {&TuneParams::cull, {0, 1}, always},
     };
     bool improved = true;
     for (int sweep = 0; sweep < 3 && improved && elapsed() < budgetSeconds; ++sweep)

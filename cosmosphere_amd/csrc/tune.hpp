@@ -22,7 +22,8 @@ struct TuneParams
     int tailLeaves;    // ... and at most this many leaves are active
     int blockScale;    // device loop: persistent density grid, resident blocks x scale / 4
     int graphPasses;   // device loop: passes per graph replay
-    int lpt;           // device loop: cost-ordered work queue
+    int lpt;           // device loop: heaviest items first in the work queue
+    int cull;          // NR whole-leaf items: per-particle culling of list entries
 };
 
 TuneParams currentParams(const State& s);

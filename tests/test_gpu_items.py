@@ -14,7 +14,7 @@ from gpu_common import DEFAULT_ITEMS, dataset, gpu_run, oracle_run, rel_err
 pytestmark = pytest.mark.gpu
 
 MAPPINGS = [("whole", {"mapping": 1}), ("flat", {"mapping": 2}), ("auto4", {"sparse": 4}),
-            ("auto64", {"sparse": 64})]
+            ("auto64", {"sparse": 64}), ("nocull", {"cull": 0}), ("nocull_whole", {"cull": 0, "mapping": 1})]
 
 
 @pytest.fixture(autouse=True)

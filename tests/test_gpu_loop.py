@@ -45,7 +45,7 @@ def same(a, b):
 
 VARIANTS = [("graph_k4", {}), ("graph_k1", {"graph_passes": 1}), ("graph_k3", {"graph_passes": 3}),
             ("direct", {"graph": 0}), ("no_lpt", {"lpt": 0}), ("overflow", {"loop_list_cap": 64}),
-            ("flat", {"mapping": 2}), ("whole", {"mapping": 1})]
+            ("flat", {"mapping": 2}), ("whole", {"mapping": 1}), ("nocull", {"cull": 0})]
 
 
 @pytest.mark.parametrize("label,opts", VARIANTS, ids=[v[0] for v in VARIANTS])
@@ -54,7 +54,7 @@ VARIANTS = [("graph_k4", {}), ("graph_k1", {"graph_passes": 1}), ("graph_k3", {"
 def test_device_loop_bit_identical(name, precision, label, opts):
     c = dataset(name)
     print("case", name, precision, label, flush=True)
-    base_opts = {k: v for k, v in opts.items() if k in ("mapping",)}
+    base_opts = {k: v for k, v in opts.items() if k in ("mapping", "cull")}
     host = solve(c, precision, loop=0, **base_opts)
     dev = solve(c, precision, loop=1, **opts)
     same(dev, host)
