@@ -168,7 +168,7 @@ class Library:
         return out
 
     PASS_FIELDS = ("leaves", "items", "iterating", "candidates", "pairs", "closure_candidates", "kernel_s",
-                   "per_particle")
+                   "per_particle", "list_s", "compact_s", "list_entries")
 
     def passes(self):
         """One row per NR pass of the last solve (columns PASS_FIELDS)."""
@@ -197,7 +197,7 @@ class Library:
 
     # ---- autotuner ----
     TUNE_OPTIONS = ("loop", "bucket", "mapping", "sparse", "tail_leaves", "loop_blocks_scale", "graph_passes", "lpt",
-                    "cull")
+                    "cull", "units")
 
     def autotune(self, n, pmass, tolh=1e-4, hfact=-1.0, periodic_box=None, budget_s=60.0):
         """Pick the solve's speed options for the uploaded set; returns the options chosen."""

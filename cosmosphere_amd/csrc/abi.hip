@@ -339,6 +339,7 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "graph_passes") s.graphPasses = std::max(1, (int)value);
     else if (nm == "lpt") s.lpt = (int)value;
     else if (nm == "cull") s.cull = (int)value;
+    else if (nm == "units") s.units = (int)value;
     else if (nm == "loop_blocks_scale") s.loopBlockScale = std::max(1, (int)value);
     else if (nm == "loop_list_cap") s.loopListCap = (long long)value;
     else
@@ -398,6 +399,7 @@ extern "C" double cosmo_ext_get_stat(const char* name)
     if (nm == "graph_passes") return s.graphPasses;
     if (nm == "lpt") return s.lpt;
     if (nm == "cull") return s.cull;
+    if (nm == "units") return s.units;
     if (nm == "mapping") return s.mapping;
     if (nm == "sparse") return s.sparse;
     if (nm == "tail_leaves") return s.tailLeaves;
@@ -413,9 +415,9 @@ extern "C" long long cosmo_ext_get_array(const char* name, void* dst, long long 
     const std::string nm(name);
     const void* src = nullptr;
     long long count = 0, esz = 8;
-    if (nm == "passes")   // host records: 8 doubles per NR pass of the last solve
+    if (nm == "passes")   // host records: one PassRecord (all doubles) per NR pass of the last solve
     {
-        const long long cnt = (long long)s.passLog.size() * 8;
+        const long long cnt = (long long)(s.passLog.size() * (sizeof(PassRecord) / sizeof(double)));
         const long long bytes = std::min(cnt * 8, maxBytes);
         if (bytes > 0) std::memcpy(dst, s.passLog.data(), (size_t)bytes);
         return cnt;

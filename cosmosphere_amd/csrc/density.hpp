@@ -65,6 +65,7 @@ struct DensArgs
     const double*   pmassSet;   // ensemble: particle mass of each set (nullptr: pmass)
     int             closure;    // converged particles pass through state 2
     int             cull;       // whole-leaf items: stage only list entries some particle's support reaches
+    int             units;      // mixed whole-leaf items: lanes over work units where that idles fewer lanes
     unsigned long long* counters;
     PassCtl*        ctl;        // device-driven pass: persistent waves pull items from a queue
     const int*      qOrder;     // ... heavy items qOrder[0, ctl->qBucket[1]) first, then every item

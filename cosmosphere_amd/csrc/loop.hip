@@ -531,7 +531,8 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
         const double tj = ran ? (double)(r.tKStart - r.tBegin) / hz : 0.0;
         const double tc = ran ? (double)(r.tEnd - r.tKEnd) / hz : (double)(r.tEnd - r.tBegin) / hz;
         s.passLog.push_back(PassRecord{(double)r.nAL, (double)r.nItems, (double)r.nIterating, (double)r.cand,
-                                       (double)r.pairs, (double)r.closCand, tk, (double)r.perParticle});
+                                       (double)r.pairs, (double)r.closCand, tk, (double)r.perParticle, tj, tc,
+                                       (double)r.listTotal});
         if (r.isNR)
         {
             t.jbuild += tj;

@@ -208,6 +208,66 @@ __device__ inline void pairMixed(float dx, float dy, float dz, float hsr, float 
     }
 }
 
+// One particle's mixed pairs with the fixed-point tile slots ptr[0], ptr[stride], ... (steps
+// of them), each slot read one step ahead: W brackets into sv, q D brackets into sp.
+__device__ inline void sweepFixed(const int4* ptr, int stride, int steps, int xq, int yq, int zq, float hsr,
+                                  float hsrLo, double& sv, float& sp, int& acc)
+{
+    int4 nx = *ptr;
+#pragma unroll 4
+    for (int t = 0; t < steps; ++t)
+    {
+        const int4 q = nx;
+        ptr += stride;
+        nx = *ptr;
+        pairMixed((float)(xq - q.x), (float)(yq - q.y), (float)(zq - q.z), hsr, hsrLo, sv, sp, acc);
+    }
+}
+
+// Work units of a whole-leaf batch of c particles.  With g = floor(64 / c) lanes per particle,
+// 64 - g c lanes idle through every tile: a third of the wavefront for c = 22, almost half for
+// c = 33.  Instead each particle's tile is cut into S units (slots s, s + S, ...: ceil(64 / S)
+// steps each) and every lane takes U consecutive units of the c S; a tile then costs U unit
+// sweeps plus a per-unit overhead instead of ceil(64 / g) steps.  S >= U, so a lane's units
+// belong to at most two particles.  The shape is chosen per c by the estimated cost (41 VALU
+// per step, 30 per unit, 8 per tile of the plain mapping), U in {2, 3}, S <= 32 (so that a
+// sweep reads no further than slot 2 S + 62 of the 128-slot tile); U = 1 keeps the plain
+// mapping.
+struct UnitShapes
+{
+    unsigned char U[kWave + 1];
+    unsigned char S[kWave + 1];
+};
+
+constexpr UnitShapes makeUnitShapes()
+{
+    UnitShapes t{};
+    constexpr int kStep = 41, kUnit = 30, kTile = 8;
+    for (int c = 1; c <= kWave; ++c)
+    {
+        const int g = kWave / c;
+        const int plain = ((kWave + g - 1) / g) * kStep + kTile;
+        int bestU = 1, bestS = g, best = plain;
+        for (int U = 2; U <= 3; ++U)
+            for (int S = U; S <= 32; ++S)
+            {
+                if (c * S > kWave * U || c * S <= kWave * (U - 1)) continue;
+                const int cost = U * (((kWave + S - 1) / S) * kStep + kUnit);
+                if (cost < best && 100 * cost < 97 * plain)
+                {
+                    best = cost;
+                    bestU = U;
+                    bestS = S;
+                }
+            }
+        t.U[c] = (unsigned char)bestU;
+        t.S[c] = (unsigned char)bestS;
+    }
+    return t;
+}
+
+__constant__ constexpr UnitShapes kUnitShapes = makeUnitShapes();
+
 // Particle sums from the accumulators: (sum W, sum -q dW/dq - 3W).
 template<bool MIXED>
 __device__ inline void particleSums(double s0, double s1, double& wsum, double& gsum)
@@ -242,6 +302,9 @@ void densityKernel(DensArgs A)
     // whole-leaf tiles: mixed = fixed-point offsets (x, y, z, pad), one 16-byte LDS read per j;
     // plain = FP64 positions
     __shared__ int4   tq[kMixed ? kNrWaves : 1][kMixed ? 2 * kWave : 1];
+    // mixed whole-leaf items in work units: the batch's particles (fixed-point offsets, hsr bits; hsrLo)
+    __shared__ int4   ptab[kMixed ? kNrWaves : 1][kMixed ? kWave : 1];
+    __shared__ float  ptabLo[kMixed ? kNrWaves : 1][kMixed ? kWave : 1];
     __shared__ double tx[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
     __shared__ double ty[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
     __shared__ double tz[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
@@ -362,21 +425,6 @@ void densityKernel(DensArgs A)
                         return true;
                     }
                 };
-                const int grp = lane / g, sub = lane - grp * g;
-                const bool has = grp < c;
-                const uint32_t i = has ? q0 + (uint32_t)rankLane[w][grp] : p0;
-                const uint8_t st = (has && MODE != 2) ? A.state[i] : 0;
-                const double xi = A.x[i], yi = A.y[i], zi = A.z[i];
-                const double hcur = has ? A.h[i] : 1.0;
-                const double hsi = 1.0 / (hcur * hcur);
-                int xq = 0, yq = 0, zq = 0;
-                if constexpr (kMixed) { xq = toFixed(xi - c0); yq = toFixed(yi - c1); zq = toFixed(zi - c2); }
-                const double hsd = hsi * invS2;
-                const float hsr = float(hsd);
-                const float hsrLo = float(hsd - (double)hsr);
-                double s0 = 0, s1 = 0;
-                int acc = 0;
-                uint32_t cand = 0;
                 auto load = [&](int l, int j) {
                     double xj = A.x[j], yj = A.y[j], zj = A.z[j];
                     if constexpr (kMixed)
@@ -396,64 +444,182 @@ void densityKernel(DensArgs A)
                     if constexpr (kMixed) tq[w][l] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);
                     else { tx[w][l] = 1e200; ty[w][l] = 1e200; tz[w][l] = 1e200; }
                 };
-                // pairs with tile slots first, first + stride, ...; each slot is read one step
-                // ahead.  Slots from nt on hold padding: [nt, 64) written with the tile, the
-                // upper half [64, 128) once per wavefront; a group of g lanes reads at most
-                // slot g - 1 + ceil(nt / g) g < 128.
-                auto sweep = [&](int first, int stride, int steps) {
-                    if constexpr (kMixed)
-                    {
-                        const int4* ptr = &tq[w][first];
-                        int4 nx = *ptr;
-                        float sp = 0.0f;
-#pragma unroll 4
-                        for (int t = 0; t < steps; ++t)
-                        {
-                            const int4 q = nx;
-                            ptr += stride;
-                            nx = *ptr;
-                            pairMixed((float)(xq - q.x), (float)(yq - q.y), (float)(zq - q.z), hsr, hsrLo, s0, sp,
-                                      acc);
-                        }
-                        s1 += (double)sp;
-                    }
-                    else
-                    {
-#pragma unroll 4
-                        for (int t = 0; t < steps; ++t)
-                        {
-                            const int k = first + t * stride;
-                            pairPlain<PER>(xi - tx[w][k], yi - ty[w][k], zi - tz[w][k], box, hsi, s0, s1, acc);
-                        }
-                    }
-                };
-                auto consume = [&](int nt) {
-                    if (g == 1)
-                    {
-                        sweep(0, 1, nt);   // list order, as the reference
-                    }
-                    else
-                    {
-                        sweep(sub, g, __builtin_amdgcn_readfirstlane((nt + g - 1) / g));
-                    }
-                    cand += sub == 0 ? (uint32_t)nt : 0u;
-                };
-                forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, pad, consume);
-                if (has)
+                const int U = (kMixed && A.units) ? (int)kUnitShapes.U[c] : 1;
+                if constexpr (kMixed)
                 {
-                    const int slot = isClosureEval<MODE>(st) ? 2 : 0;
-                    cc[slot] += (unsigned long long)acc;
-                    cc[slot + 1] += cand;
+                    if (U > 1)
+                    {
+                        // ---- work units: each particle's tile in S units, U consecutive units
+                        // per lane, a lane's units summed per particle (A: its first, B: the
+                        // next), the particle's sums gathered from its lanes at the end ----
+                        const int S = kUnitShapes.S[c];
+                        const int nU = c * S;
+                        const uint32_t mS = (65536u + (uint32_t)S - 1u) / (uint32_t)S;   // u / S = (u mS) >> 16, u < 1024
+                        const bool own = lane < c;   // lane r holds particle r of the batch
+                        const uint32_t i = own ? q0 + (uint32_t)rankLane[w][lane] : p0;
+                        const uint8_t st = (own && MODE != 2) ? A.state[i] : 0;
+                        const double hcur = own ? A.h[i] : 1.0;
+                        if (own)
+                        {
+                            const double hsd = (1.0 / (hcur * hcur)) * invS2;
+                            const float hsr = float(hsd);
+                            ptab[w][lane] = make_int4(toFixed(A.x[i] - c0), toFixed(A.y[i] - c1), toFixed(A.z[i] - c2),
+                                                      __float_as_int(hsr));
+                            ptabLo[w][lane] = float(hsd - (double)hsr);
+                        }
+                        waveLdsSync();
+                        const int u0 = lane * U;
+                        const int pA = (int)(((uint32_t)u0 * mS) >> 16);
+                        double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+                        int accA = 0, accB = 0;
+                        uint32_t candA = 0, candB = 0;
+                        auto consumeUnits = [&](int nt) {
+                            const int steps = __builtin_amdgcn_readfirstlane((nt + S - 1) / S);
+                            for (int k = 0; k < U; ++k)
+                            {
+                                const int u = u0 + k;
+                                if (u < nU)
+                                {
+                                    const int pu = (int)(((uint32_t)u * mS) >> 16);
+                                    const int sg = u - pu * S;
+                                    const int4 pp = ptab[w][pu];
+                                    double sv = 0.0;
+                                    float sp = 0.0f;
+                                    int acc = 0;
+                                    sweepFixed(&tq[w][sg], S, steps, pp.x, pp.y, pp.z, __int_as_float(pp.w),
+                                               ptabLo[w][pu], sv, sp, acc);
+                                    const uint32_t cn = sg == 0 ? (uint32_t)nt : 0u;
+                                    if (pu == pA) { a0 += sv; a1 += (double)sp; accA += acc; candA += cn; }
+                                    else          { b0 += sv; b1 += (double)sp; accB += acc; candB += cn; }
+                                }
+                            }
+                        };
+                        forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, pad, consumeUnits);
+                        // counters by the state of each lane's particles, read before any update
+                        if (u0 < nU)
+                        {
+                            const int pB = (int)(((uint32_t)min(u0 + U - 1, nU - 1) * mS) >> 16);
+                            const uint8_t sA = MODE != 2 ? A.state[q0 + (uint32_t)rankLane[w][pA]] : 0;
+                            const uint8_t sB = MODE != 2 ? A.state[q0 + (uint32_t)rankLane[w][pB]] : 0;
+                            const int slotA = isClosureEval<MODE>(sA) ? 2 : 0, slotB = isClosureEval<MODE>(sB) ? 2 : 0;
+                            cc[slotA] += (unsigned long long)accA;
+                            cc[slotA + 1] += candA;
+                            cc[slotB] += (unsigned long long)accB;
+                            cc[slotB + 1] += candB;
+                        }
+                        // each lane's partial sums through the tile area (free until the next
+                        // batch); particle r's units lie on lanes floor(r S / U) .. floor((r S + S - 1) / U)
+                        // slots 2 l, 2 l + 1 of the 128: lane l's (a0, a1), (b0, b1) as bit patterns
+                        auto pack = [](double x, double y) {
+                            const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+                            return make_int4((int)bx, (int)(bx >> 32), (int)by, (int)(by >> 32));
+                        };
+                        auto lo64 = [](const int4& v) {
+                            return __longlong_as_double(((long long)v.y << 32) | (long long)(uint32_t)v.x);
+                        };
+                        auto hi64 = [](const int4& v) {
+                            return __longlong_as_double(((long long)v.w << 32) | (long long)(uint32_t)v.z);
+                        };
+                        waveLdsSync();
+                        tq[w][2 * lane] = pack(a0, a1);
+                        tq[w][2 * lane + 1] = pack(b0, b1);
+                        waveLdsSync();
+                        double s0 = 0, s1 = 0;
+                        if (own)
+                        {
+                            const int l0 = (lane * S) / U, l1 = (lane * S + S - 1) / U;
+                            for (int l = l0; l <= l1; ++l)
+                            {
+                                const int pl = (int)(((uint32_t)(l * U) * mS) >> 16);
+                                const int4 v = tq[w][2 * l + (pl == lane ? 0 : 1)];
+                                s0 += lo64(v);
+                                s1 += hi64(v);
+                            }
+                        }
+                        waveLdsSync();
+                        tq[w][kWave + lane] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);   // the padding back
+                        waveLdsSync();
+                        double wsum, gsum;
+                        particleSums<kMixed>(s0, s1, wsum, gsum);
+                        if (own) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
+                        continue;
+                    }
                 }
-                // group sums, a tree over lanes sub .. g-1 of each group (lane sub = 0 ends with it)
-                for (int off = 1; off < g; off <<= 1)
                 {
-                    const double t0 = __shfl_down(s0, off, kWave), t1 = __shfl_down(s1, off, kWave);
-                    if (sub + off < g) { s0 += t0; s1 += t1; }
+                    const int grp = lane / g, sub = lane - grp * g;
+                    const bool has = grp < c;
+                    const uint32_t i = has ? q0 + (uint32_t)rankLane[w][grp] : p0;
+                    const uint8_t st = (has && MODE != 2) ? A.state[i] : 0;
+                    const double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+                    const double hcur = has ? A.h[i] : 1.0;
+                    const double hsi = 1.0 / (hcur * hcur);
+                    int xq = 0, yq = 0, zq = 0;
+                    if constexpr (kMixed) { xq = toFixed(xi - c0); yq = toFixed(yi - c1); zq = toFixed(zi - c2); }
+                    const double hsd = hsi * invS2;
+                    const float hsr = float(hsd);
+                    const float hsrLo = float(hsd - (double)hsr);
+                    double s0 = 0, s1 = 0;
+                    int acc = 0;
+                    uint32_t cand = 0;
+                    // pairs with tile slots first, first + stride, ...; each slot is read one step
+                    // ahead.  Slots from nt on hold padding: [nt, 64) written with the tile, the
+                    // upper half [64, 128) once per wavefront; a group of g lanes reads at most
+                    // slot g - 1 + ceil(nt / g) g < 128.
+                    auto sweep = [&](int first, int stride, int steps) {
+                        if constexpr (kMixed)
+                        {
+                            const int4* ptr = &tq[w][first];
+                            int4 nx = *ptr;
+                            float sp = 0.0f;
+    #pragma unroll 4
+                            for (int t = 0; t < steps; ++t)
+                            {
+                                const int4 q = nx;
+                                ptr += stride;
+                                nx = *ptr;
+                                pairMixed((float)(xq - q.x), (float)(yq - q.y), (float)(zq - q.z), hsr, hsrLo, s0, sp,
+                                          acc);
+                            }
+                            s1 += (double)sp;
+                        }
+                        else
+                        {
+    #pragma unroll 4
+                            for (int t = 0; t < steps; ++t)
+                            {
+                                const int k = first + t * stride;
+                                pairPlain<PER>(xi - tx[w][k], yi - ty[w][k], zi - tz[w][k], box, hsi, s0, s1, acc);
+                            }
+                        }
+                    };
+                    auto consume = [&](int nt) {
+                        if (g == 1)
+                        {
+                            sweep(0, 1, nt);   // list order, as the reference
+                        }
+                        else
+                        {
+                            sweep(sub, g, __builtin_amdgcn_readfirstlane((nt + g - 1) / g));
+                        }
+                        cand += sub == 0 ? (uint32_t)nt : 0u;
+                    };
+                    forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, pad, consume);
+                    if (has)
+                    {
+                        const int slot = isClosureEval<MODE>(st) ? 2 : 0;
+                        cc[slot] += (unsigned long long)acc;
+                        cc[slot + 1] += cand;
+                    }
+                    // group sums, a tree over lanes sub .. g-1 of each group (lane sub = 0 ends with it)
+                    for (int off = 1; off < g; off <<= 1)
+                    {
+                        const double t0 = __shfl_down(s0, off, kWave), t1 = __shfl_down(s1, off, kWave);
+                        if (sub + off < g) { s0 += t0; s1 += t1; }
+                    }
+                    double wsum, gsum;
+                    particleSums<kMixed>(s0, s1, wsum, gsum);
+                    if (has && sub == 0) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
                 }
-                double wsum, gsum;
-                particleSums<kMixed>(s0, s1, wsum, gsum);
-                if (has && sub == 0) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
             }
         }
     };

@@ -128,6 +128,7 @@ static DensArgs densArgs(State& s, const int* leaves, int nLeaves, double pmass,
     a.counters = counters;
     a.nItems = nLeaves;   // whole-leaf items unless the caller sets the item arrays
     a.cull = s.cull;
+    a.units = s.units;
     return a;
 }
 
@@ -250,7 +251,8 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         compact(nAL, ctr, &a3);
         const double tj = secs(a0, a1), tk = secs(a1, a2), tc = secs(a2, a3);
         s.passLog.push_back(PassRecord{(double)nALpass, (double)da.nItems, (double)nIterating, (double)c[kCtrCand],
-                                       (double)c[kCtrPairs], (double)c[kCtrClosCand], tk, (double)perParticle});
+                                       (double)c[kCtrPairs], (double)c[kCtrClosCand], tk, (double)perParticle,
+                                       tj, tc, (double)s.listTotal});
         if (nrPass)
         {
             t.jbuild += tj;

@@ -59,6 +59,9 @@ struct Timings
 struct PassRecord
 {
     double leaves, items, iterating, candidates, pairs, closureCandidates, kernelSeconds, perParticle;
+    double listSeconds;      // pass start to kernel start: hmax, lists, items
+    double compactSeconds;   // kernel end to pass end: compaction
+    double listEntries;      // entries of the pass's lists
 };
 
 struct ForceTimings
@@ -159,6 +162,7 @@ struct State
     int      graphPasses = 4;      // passes per graph replay (per host round trip)
     int      lpt = 1;              // work queue: the heaviest items first
     int      cull = 1;             // NR whole-leaf items: per-particle culling of list entries
+    int      units = 1;            // mixed NR whole-leaf items: lanes over work units where that idles fewer lanes
     int      loopBlockScale = 4;   // persistent density grid: resident blocks x scale / 4
     int      loopBlocks = 0, loopBlocksPrecision = -1, loopBlocksPeriodic = -1;
     long long loopListCap = 0;     // verification: starting list capacity of the device-driven loop (0: auto)

@@ -14,6 +14,7 @@
  *   graphPasses   passes per graph replay (idle passes at the end vs host round trips)
  *   lpt           heaviest items first in the work queue
  *   cull          per-particle culling of the NR whole-leaf items' list entries
+ *   units         mixed whole-leaf items: lanes over work units (fewer idle lanes)
  *
  * by a coordinate search: one option at a time, each candidate value timed by a solve (a second
  * solve confirms a candidate within 5% of the best), the best kept, sweeps repeated until
@@ -36,7 +37,7 @@ namespace csph
 TuneParams currentParams(const State& s)
 {
     return TuneParams{s.loopMode, s.bucket, s.mapping, s.sparse, s.tailLeaves, s.loopBlockScale, s.graphPasses, s.lpt,
-                      s.cull};
+                      s.cull, s.units};
 }
 
 void applyParams(State& s, const TuneParams& p)
@@ -50,23 +51,25 @@ void applyParams(State& s, const TuneParams& p)
     s.graphPasses = p.graphPasses;
     s.lpt = p.lpt;
     s.cull = p.cull;
+    s.units = p.units;
 }
 
 std::string describe(const TuneParams& p)
 {
     char b[256];
     std::snprintf(b, sizeof(b), "loop=%d bucket=%d mapping=%d sparse=%d tail_leaves=%d loop_blocks_scale=%d "
-                                "graph_passes=%d lpt=%d cull=%d",
-                  p.loop, p.bucket, p.mapping, p.sparse, p.tailLeaves, p.blockScale, p.graphPasses, p.lpt, p.cull);
+                                "graph_passes=%d lpt=%d cull=%d units=%d",
+                  p.loop, p.bucket, p.mapping, p.sparse, p.tailLeaves, p.blockScale, p.graphPasses, p.lpt, p.cull,
+                  p.units);
     return b;
 }
 
 static bool parse(const std::string& text, TuneParams& p)
 {
     return std::sscanf(text.c_str(), "loop=%d bucket=%d mapping=%d sparse=%d tail_leaves=%d loop_blocks_scale=%d "
-                                     "graph_passes=%d lpt=%d cull=%d",
+                                     "graph_passes=%d lpt=%d cull=%d units=%d",
                        &p.loop, &p.bucket, &p.mapping, &p.sparse, &p.tailLeaves, &p.blockScale, &p.graphPasses,
-                       &p.lpt, &p.cull) == 9;
+                       &p.lpt, &p.cull, &p.units) == 10;
 }
 
 namespace
@@ -197,6 +200,7 @@ TuneResult autotune(State& s, const std::string& signature, const std::function<
         {&TuneParams::graphPasses, {4, 1, 2, 8}, deviceLoopOn},
         {&TuneParams::lpt, {1, 0}, deviceLoopOn},
         {&TuneParams::cull, {0, 1}, always},
+        {&TuneParams::units, {1, 0}, always},
     };
     bool improved = true;
     for (int sweep = 0; sweep < 3 && improved && elapsed() < budgetSeconds; ++sweep)

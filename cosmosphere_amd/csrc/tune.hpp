@@ -24,6 +24,7 @@ struct TuneParams
     int graphPasses;   // device loop: passes per graph replay
     int lpt;           // device loop: heaviest items first in the work queue
     int cull;          // NR whole-leaf items: per-particle culling of list entries
+    int units;         // mixed NR whole-leaf items: lanes over work units
 };
 
 TuneParams currentParams(const State& s);

@@ -94,7 +94,10 @@ int cosmo_ext_solve_dens_h(int n, double pmass, int mode, int withGradients, int
  * "loop" (NR passes: 0 host-driven, 1 device-driven, the default: counts in device memory,
  * "graph_passes" passes per HIP graph replay when "graph" is 1, persistent density grid of
  * "loop_blocks_scale" / 4 times the resident blocks, cost-ordered work queue when "lpt" is 1),
- * "bucket" (leaf bucket size, default 64), "cull" (1, the default: NR whole-leaf items stage only the
+ * "bucket" (leaf bucket size, default 64), "units" (1, the default: mixed NR whole-leaf items cut
+ * each particle's tile into work units spread evenly over the 64 lanes where one particle per
+ * lane group would leave lanes idle; 0: one group of floor(64 / c) lanes per particle),
+ * "cull" (1, the default: NR whole-leaf items stage only the
  * list entries some particle's own support reaches; 0: every entry of the tight list),
  * "mapping" (NR work items: 0 auto = one wavefront per leaf, one per particle once at most
  * "sparse" particles of the leaf remain; 1 = always one per leaf; 2 = always one per
@@ -132,7 +135,7 @@ int cosmo_batch_solve(const double* pmass, int periodic, const double* boxes, do
  * options the results do not depend on -- NR loop driver ("loop"), leaf bucket size ("bucket"),
  * wave-per-leaf vs flat item mapping ("mapping", "sparse", "tail_leaves"), persistent grid size
  * ("loop_blocks_scale"), passes per graph replay ("graph_passes"), heavy items first ("lpt"),
- * per-particle culling of list entries ("cull")
+ * per-particle culling of list entries ("cull"), lanes over work units ("units")
  * -- one option at a time, and leaves the fastest in force.  The uploaded inputs are put back
  * afterwards (a following solve starts from them; download nothing before it).  budgetSeconds
  * bounds the search.  A workload with the same signature (particle count, set sizes, kernel,

@@ -4,7 +4,12 @@ Every mapping (whole-leaf items, per-particle items, the automatic switch at sev
 thresholds) and the j-leaf pruning that goes with it must leave the plain FP64 solve within
 1e-10 of the oracle with the same iteration count and the same accepted pairs (pruning only
 drops leaves without a pair inside the support), and the mixed solve within its 1e-6 bar.
-With the closure on, rho is the oracle's closure at the returned h and h is unchanged."""
+With the closure on, rho is the oracle's closure at the returned h and h is unchanged.
+
+Work units (mixed whole-leaf items): lanes over (particle, tile-slot subset) units instead of
+one group of floor(64 / c) lanes per particle changes only the order in which each particle's
+pair terms are summed, so the first pass (identical h and lists) has identical candidate and
+accepted-pair counts with the units on and off, and both solves stay within the 1e-6 bar."""
 import numpy as np
 import pytest
 
@@ -51,6 +56,24 @@ def test_mappings_mixed(label, opts):
     assert rel_err(lib.array("h")[:na], o.get("h")[:na]) < 1e-6
     o.closure(c["pmass"])
     assert rel_err(lib.array("rho")[:na], o.get("rho")[:na]) < 1e-6
+
+
+@pytest.mark.parametrize("mapping", [0, 1])
+@pytest.mark.parametrize("name", ["plummer_cubic", "plummer_quintic", "dead", "C1small"])
+def test_work_units_mixed(name, mapping):
+    c = dataset(name)
+    print("case units", name, mapping, flush=True)
+    o, _ = oracle_run(c, gradients=False)
+    na = o.geti("nAlive")
+    o.closure(c["pmass"])
+    first = {}
+    for units in (1, 0):
+        lib, _ = gpu_run(c, precision=1, gradients=False, mapping=mapping, units=units)
+        assert rel_err(lib.array("h")[:na], o.get("h")[:na]) < 1e-6
+        assert rel_err(lib.array("rho")[:na], o.get("rho")[:na]) < 1e-6
+        p = lib.passes()
+        first[units] = (p[0, 3], p[0, 4])
+    assert first[0] == first[1]
 
 
 @pytest.mark.parametrize("name", ["plummer_cubic", "plummer_quintic"])

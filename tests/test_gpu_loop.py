@@ -45,7 +45,8 @@ def same(a, b):
 
 VARIANTS = [("graph_k4", {}), ("graph_k1", {"graph_passes": 1}), ("graph_k3", {"graph_passes": 3}),
             ("direct", {"graph": 0}), ("no_lpt", {"lpt": 0}), ("overflow", {"loop_list_cap": 64}),
-            ("flat", {"mapping": 2}), ("whole", {"mapping": 1}), ("nocull", {"cull": 0})]
+            ("flat", {"mapping": 2}), ("whole", {"mapping": 1}), ("nocull", {"cull": 0}),
+            ("nounits", {"units": 0})]
 
 
 @pytest.mark.parametrize("label,opts", VARIANTS, ids=[v[0] for v in VARIANTS])
@@ -54,7 +55,7 @@ VARIANTS = [("graph_k4", {}), ("graph_k1", {"graph_passes": 1}), ("graph_k3", {"
 def test_device_loop_bit_identical(name, precision, label, opts):
     c = dataset(name)
     print("case", name, precision, label, flush=True)
-    base_opts = {k: v for k, v in opts.items() if k in ("mapping", "cull")}
+    base_opts = {k: v for k, v in opts.items() if k in ("mapping", "cull", "units")}
     host = solve(c, precision, loop=0, **base_opts)
     dev = solve(c, precision, loop=1, **opts)
     same(dev, host)
@@ -68,6 +69,11 @@ def test_device_loop_oracle_and_graph_reuse():
     lib, iters = gpu_run(c, precision=0, gradients=False)
     o, unconv = oracle_run(c, gradients=False)
     assert unconv == 0 and iters == o.geti("itersRun")
+    # the pass records add up to the solve's counters
+    p = lib.passes()
+    assert p.shape[1] == len(lib.PASS_FIELDS)
+    assert p[:, 3].sum() == lib.stat("candidates") and p[:, 4].sum() == lib.stat("pairs")
+    assert p[0, 10] > 0 and (p[:, 8] >= 0).all()
     na = o.geti("nAlive")
     assert rel_err(lib.array("h")[:na], o.get("h")[:na]) < 1e-10
     assert rel_err(lib.array("rho")[:na], o.get("rho")[:na]) < 1e-10
