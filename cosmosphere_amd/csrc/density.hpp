@@ -37,7 +37,7 @@ struct PassCtl
 struct PassLogDev
 {
     unsigned long long nAL, nItems, nIterating, cand, pairs, closCand, closPairs;
-    unsigned long long tBegin, tKStart, tKEnd, tEnd, perParticle, listTotal, isNR, pad0, pad1;
+    unsigned long long tBegin, tKStart, tKEnd, tEnd, perParticle, listTotal, isNR, maxListLog2, heavyItems;
 };
 
 struct DensArgs

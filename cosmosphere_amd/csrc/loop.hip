@@ -180,6 +180,8 @@ __global__ __launch_bounds__(256) void passEndKernel(PassCtl* __restrict__ C,
         r.perParticle = C->perParticle;
         r.listTotal = C->listNeed;
         r.isNR = C->isNR;
+        r.maxListLog2 = C->qBucket[0];
+        r.heavyItems = C->qBucket[1];
     }
     C->passes += 1u;
     if (C->isNR) C->iters += 1u;
@@ -532,7 +534,7 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
         const double tc = ran ? (double)(r.tEnd - r.tKEnd) / hz : (double)(r.tEnd - r.tBegin) / hz;
         s.passLog.push_back(PassRecord{(double)r.nAL, (double)r.nItems, (double)r.nIterating, (double)r.cand,
                                        (double)r.pairs, (double)r.closCand, tk, (double)r.perParticle, tj, tc,
-                                       (double)r.listTotal});
+                                       (double)r.listTotal, (double)r.maxListLog2, (double)r.heavyItems});
         if (r.isNR)
         {
             t.jbuild += tj;

@@ -62,6 +62,8 @@ struct PassRecord
     double listSeconds;      // pass start to kernel start: hmax, lists, items
     double compactSeconds;   // kernel end to pass end: compaction
     double listEntries;      // entries of the pass's lists
+    double maxListLog2;      // device-driven passes: floor(log2) of the longest list (with the cost order on)
+    double heavyItems;       // device-driven passes: items taken first by the cost order
 };
 
 struct ForceTimings
