@@ -342,6 +342,8 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "units") s.units = (int)value;
     else if (nm == "loop_blocks_scale") s.loopBlockScale = std::max(1, (int)value);
     else if (nm == "loop_list_cap") s.loopListCap = (long long)value;
+    else if (nm == "list_heavy_steps") s.listHeavySteps = (int)value;
+    else if (nm == "list_expand_leaves") s.listExpandLeaves = (int)value;
     else
     {
         std::fprintf(stderr, "FATAL: cosmo_ext_set_option: unknown option '%s'\n", name);

@@ -122,6 +122,22 @@ constexpr int kListThreads = 256;
 constexpr int kGroups      = kListThreads / 8;   // i-leaves per block
 constexpr int kStack       = 152;                // >= 7*20 + 8
 
+// Long walks.  A group walks one list sequentially, one node per step, each step a chain of
+// dependent loads; a list of tens of thousands of entries (a leaf whose particles still to
+// evaluate reach into a dense region) then takes milliseconds while the rest of the chip
+// idles.  A count walk that passes kHeavySteps steps hands its list to a whole block: the
+// list is out(root), where out(N) = N's accepted leaf children in octant order followed by
+// out(C) of its accepted internal children C in reverse octant order (the LIFO order of the
+// walk).  The block rewrites out(root) as a sequence of tokens (a leaf, or out(C) of a node
+// C), expanding every out(C) of a node with more than kExpandLeaves leaves while the token
+// buffer holds the result; the groups then walk the out(C) tokens at once, count them, scan
+// the token sizes and fill each token at its offset.  The list is the sequential walk's,
+// entry for entry.
+constexpr int kHeavySteps   = 512;    // per 32768 lists of the launch (a launch of many lists
+                                      // keeps the chip busy through long walks)
+constexpr int kTokCap       = 2048;
+constexpr int kExpandLeaves = 256;
+
 struct ListArgs
 {
     const int*      leaves;
@@ -142,42 +158,49 @@ struct ListArgs
     int*            ovf;
     int             nNodes;   // walk cap: a tree walk visits each node at most once
     const PassCtl*  ctl;      // device-driven pass: ctl->nAL leaves, nothing when ctl->skip
+    const int*      nodeLeaves;   // long walks: leaves under each internal node
+    int*            heavyN;       // ... lists handed to the block walk (count), nullptr: none
+    int*            heavyList;    // ... their positions
+    uint8_t*        heavyFlag;    // ... per position: 1 if the block walk owns the list
+    int             heavySteps;   // ... hand-over threshold per 32768 lists (kHeavySteps)
+    int             expandLeaves; // ... expansion threshold (kExpandLeaves)
 };
 
-template<bool SYM, bool COUNT, bool PER>
-__global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
+// The accept test of one i-leaf's list (include/tree.cuh:228-252 and the tight test).
+template<bool SYM, bool PER>
+struct Reach
 {
-#pragma clang fp contract(off)
-    __shared__ int stk[kGroups][kStack];
-    const int g = threadIdx.x >> 3, o = threadIdx.x & 7;
-    if (A.ctl && A.ctl->skip) return;
-    const int nL = A.ctl ? (int)A.ctl->nAL : A.nLeaves;
-    const int shift = (laneId() >> 3) << 3;
-    const uint32_t below = (1u << o) - 1u;
-    for (int a = blockIdx.x * kGroups + g; a < nL; a += gridDim.x * kGroups)
-    {
+    double ic[3], is[3], aSize[3], pad[3];
+    double radHi, tight2;
+    DBox   box;
 
-    const int iLeaf = A.leaves ? A.leaves[a] : a;
-    const int in = A.leafNode[iLeaf];
-    const int b = A.leafSet[iLeaf];
-    const DBox box = A.boxes[b];
-    const double ic[3] = {A.cx[in], A.cy[in], A.cz[in]};
-    const double is[3] = {A.sx[in], A.sy[in], A.sz[in]};
-    double hmax = A.hmaxLeaf[iLeaf];
-    if (!isfinite(hmax) || hmax <= 0.0) hmax = 0.0;
-    const double radHi = kRad * hmax;
-    const double aSize[3] = {SYM ? is[0] : is[0] + radHi, SYM ? is[1] : is[1] + radHi, SYM ? is[2] : is[2] + radHi};
-
-    // tight-list reach, squared (negative: no second test)
-    double tight2 = -1.0;
-    if (!SYM && A.hRemLeaf)
+    __device__ Reach(const ListArgs& A, int iLeaf)
     {
-        const double hr = A.hRemLeaf[iLeaf];
-        tight2 = (kRad * hr) * (kRad * hr) * (1.0 + 1e-5);
+        const int in = A.leafNode[iLeaf];
+        box = A.boxes[A.leafSet[iLeaf]];
+        ic[0] = A.cx[in]; ic[1] = A.cy[in]; ic[2] = A.cz[in];
+        is[0] = A.sx[in]; is[1] = A.sy[in]; is[2] = A.sz[in];
+        double hmax = A.hmaxLeaf[iLeaf];
+        if (!isfinite(hmax) || hmax <= 0.0) hmax = 0.0;
+        radHi = kRad * hmax;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+        {
+            aSize[a] = SYM ? is[a] : is[a] + radHi;
+            pad[a] = 2.0 * kFinestCell * box.l[a];
+        }
+        // tight-list reach, squared (negative: no second test)
+        tight2 = -1.0;
+        if (!SYM && A.hRemLeaf)
+        {
+            const double hr = A.hRemLeaf[iLeaf];
+            tight2 = (kRad * hr) * (kRad * hr) * (1.0 + 1e-5);
+        }
     }
-    const double pad[3] = {2.0 * kFinestCell * box.l[0], 2.0 * kFinestCell * box.l[1], 2.0 * kFinestCell * box.l[2]};
 
-    auto inRange = [&](int node) -> bool {
+    __device__ bool operator()(const ListArgs& A, int node) const
+    {
+#pragma clang fp contract(off)
         const double d2 = boxDist2<PER>(ic, aSize, A.cx[node], A.cy[node], A.cz[node], A.sx[node], A.sy[node],
                                         A.sz[node], box);
         if constexpr (SYM)
@@ -194,38 +217,31 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
                                             A.sz[node], box);
             return g2 <= tight2;
         }
-    };
-
-    const uint32_t base = COUNT ? 0u : A.jOffset[a];
-    const int root = A.nodeFirst[b];
-    if (!inRange(root))
-    {
-        if (COUNT && o == 0) A.jCount[a] = 0;
-        continue;
     }
-    const int rc = A.nodeChild[root];
-    if (rc < 0)
-    {
-        if (o == 0)
-        {
-            if (COUNT) A.jCount[a] = 1;
-            else       A.jList[base] = ~rc;
-        }
-        continue;
-    }
+};
 
-    const uint32_t limit = COUNT ? 0u : A.jOffset[a + 1];
+// out(root) of an accepted internal node by the 8 lanes of a group (lane o tests child o of
+// the current node): the count, and without COUNT the entries at jList[base ...] below limit.
+// Stops after stepCap steps with capped set.
+template<bool COUNT, class R>
+__device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int root, int* __restrict__ stk,
+                                     uint32_t base, uint32_t limit, int stepCap, bool& capped)
+{
+    const int o = threadIdx.x & 7;
+    const int shift = (laneId() >> 3) << 3;
+    const uint32_t below = (1u << o) - 1u;
     uint32_t count = 0;
     int sp = 0, node = root, steps = 0;
+    capped = false;
     while (true)
     {
-        if (++steps > A.nNodes)
+        if (++steps > stepCap)
         {
-            if (o == 0) atomicAdd(&A.ovf[1], 1);
+            capped = true;
             break;
         }
         const int  c    = A.nodeChild[node] + o;
-        const bool acc  = inRange(c);
+        const bool acc  = inRange(A, c);
         const int  cref = A.nodeChild[c];
         const bool isLeaf = acc && cref < 0;
         const bool isInt  = acc && cref >= 0;
@@ -241,7 +257,7 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
         const int push = __popc(mI);
         if (sp + push <= kStack)
         {
-            if (isInt) stk[g][sp + __popc(mI & below)] = c;
+            if (isInt) stk[sp + __popc(mI & below)] = c;
             sp += push;
         }
         else if (o == 0)
@@ -250,10 +266,211 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
         }
         waveLdsSync();
         if (sp == 0) break;
-        node = stk[g][--sp];
+        node = stk[--sp];
         waveLdsSync();
     }
-    if (COUNT && o == 0) A.jCount[a] = count;
+    return count;
+}
+
+template<bool SYM, bool COUNT, bool PER>
+__global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
+{
+    __shared__ int stk[kGroups][kStack];
+    const int g = threadIdx.x >> 3, o = threadIdx.x & 7;
+    if (A.ctl && A.ctl->skip) return;
+    const int nL = A.ctl ? (int)A.ctl->nAL : A.nLeaves;
+    const bool heavy = A.heavyList != nullptr;
+    const int heavyCap = A.heavySteps * max(1, (nL + 32767) / 32768);
+    for (int a = blockIdx.x * kGroups + g; a < nL; a += gridDim.x * kGroups)
+    {
+        if (!COUNT && heavy && A.heavyFlag[a]) continue;   // filled by the block walk
+        const int iLeaf = A.leaves ? A.leaves[a] : a;
+        const Reach<SYM, PER> inRange(A, iLeaf);
+        const uint32_t base = COUNT ? 0u : A.jOffset[a];
+        const int root = A.nodeFirst[A.leafSet[iLeaf]];
+        if (COUNT && heavy && o == 0) A.heavyFlag[a] = 0;
+        if (!inRange(A, root))
+        {
+            if (COUNT && o == 0) A.jCount[a] = 0;
+            continue;
+        }
+        const int rc = A.nodeChild[root];
+        if (rc < 0)
+        {
+            if (o == 0)
+            {
+                if (COUNT) A.jCount[a] = 1;
+                else       A.jList[base] = ~rc;
+            }
+            continue;
+        }
+        bool capped;
+        const uint32_t count = groupWalk<COUNT>(A, inRange, root, stk[g], base, COUNT ? 0u : A.jOffset[a + 1],
+                                                (COUNT && heavy) ? heavyCap : A.nNodes, capped);
+        if (capped)
+        {
+            if (COUNT && heavy)
+            {
+                if (o == 0)
+                {
+                    A.heavyList[atomicAdd(A.heavyN, 1)] = a;
+                    A.heavyFlag[a] = 1;
+                }
+                continue;
+            }
+            if (o == 0) atomicAdd(&A.ovf[1], 1);
+        }
+        if (COUNT && o == 0) A.jCount[a] = count;
+    }
+}
+
+// Block-wide exclusive scan of v[0, n) in place (n <= kTokCap); returns the total.
+__device__ inline uint32_t blockScan(uint32_t* __restrict__ v, int n, uint32_t* __restrict__ red)
+{
+    constexpr int kPer = kTokCap / kListThreads;
+    const int t = threadIdx.x;
+    uint32_t loc[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+    {
+        const int i = t * kPer + k;
+        loc[k] = i < n ? v[i] : 0u;
+        sum += loc[k];
+    }
+    const uint32_t incl = waveInclusiveScan(sum);
+    if (laneId() == kWave - 1) red[t >> 6] = incl;
+    __syncthreads();
+    uint32_t before = incl - sum;
+    for (int w = 0; w < (t >> 6); ++w) before += red[w];
+    const uint32_t total = red[0] + red[1] + red[2] + red[3];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+    {
+        const int i = t * kPer + k;
+        if (i < n) v[i] = before;
+        before += loc[k];
+    }
+    __syncthreads();
+    return total;
+}
+
+// The lists that count walks handed over (heavyList[0, *heavyN)), one block per list.
+template<bool SYM, bool COUNT, bool PER>
+__global__ __launch_bounds__(kListThreads) void jlistHeavyKernel(ListArgs A)
+{
+    static_assert(kListThreads == 4 * kWave, "blockScan assumes four wavefronts");
+    __shared__ int stk[kGroups][kStack];
+    __shared__ int tok[2][kTokCap];       // >= 0: a leaf; < 0: ~C, the list out(C) of node C
+    __shared__ uint32_t tmask[kTokCap];   // expansion: bit 16 expanded, bits 0-7 leaf, 8-15 internal children
+    __shared__ uint32_t toff[kTokCap];    // token sizes, then offsets
+    __shared__ uint32_t red[kListThreads / kWave];
+    __shared__ int nNew;
+    if (A.ctl && A.ctl->skip) return;
+    const int g = threadIdx.x >> 3, o = threadIdx.x & 7;
+    const int shift = (laneId() >> 3) << 3;
+    const uint32_t below = (1u << o) - 1u;
+    const int nH = *A.heavyN;
+    for (int k = blockIdx.x; k < nH; k += gridDim.x)
+    {
+        const int a = A.heavyList[k];
+        const int iLeaf = A.leaves ? A.leaves[a] : a;
+        const Reach<SYM, PER> inRange(A, iLeaf);
+        const int root = A.nodeFirst[A.leafSet[iLeaf]];   // internal and accepted: its walk was long
+        int cur = 0, nTok = 1, ne = 1;   // ne: out(C) tokens made by the last round (bounds the next)
+        if (threadIdx.x == 0) tok[0][0] = ~root;
+        __syncthreads();
+        while (ne > 0 && nTok + 7 * ne <= kTokCap)
+        {
+            for (int t = g; t < nTok; t += kGroups)
+            {
+                const int v = tok[cur][t];
+                uint32_t m = 0;
+                if (v < 0 && A.nodeLeaves[~v] > A.expandLeaves)
+                {
+                    const int c = A.nodeChild[~v] + o;
+                    const bool acc = inRange(A, c);
+                    const int cref = A.nodeChild[c];
+                    const uint32_t mL = (uint32_t)(ballot64(acc && cref < 0) >> shift) & 0xFFu;
+                    const uint32_t mI = (uint32_t)(ballot64(acc && cref >= 0) >> shift) & 0xFFu;
+                    m = 0x10000u | mL | (mI << 8);
+                }
+                if (o == 0) tmask[t] = m;
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < nTok; t += kListThreads)
+            {
+                const uint32_t m = tmask[t];
+                toff[t] = (m & 0x10000u) ? (uint32_t)(__popc(m & 0xFFu) + __popc((m >> 8) & 0xFFu)) : 1u;
+            }
+            if (threadIdx.x == 0) nNew = 0;
+            __syncthreads();
+            const int n2 = (int)blockScan(toff, nTok, red);
+            for (int t = g; t < nTok; t += kGroups)
+            {
+                const int v = tok[cur][t];
+                const uint32_t m = tmask[t], at = toff[t];
+                if (!(m & 0x10000u))
+                {
+                    if (o == 0) tok[cur ^ 1][at] = v;
+                }
+                else
+                {
+                    const uint32_t mL = m & 0xFFu, mI = (m >> 8) & 0xFFu;
+                    const int c = A.nodeChild[~v] + o;
+                    if ((mL >> o) & 1u) tok[cur ^ 1][at + __popc(mL & below)] = ~A.nodeChild[c];
+                    if ((mI >> o) & 1u) tok[cur ^ 1][at + __popc(mL) + __popc(mI >> (o + 1))] = ~c;
+                    if (o == 0 && mI) atomicAdd(&nNew, (int)__popc(mI));
+                }
+            }
+            __syncthreads();
+            cur ^= 1;
+            nTok = n2;
+            ne = nNew;
+            __syncthreads();
+        }
+        // token sizes: a leaf 1, out(C) by a group walk
+        for (int t = g; t < nTok; t += kGroups)
+        {
+            const int v = tok[cur][t];
+            uint32_t n = 1;
+            if (v < 0)
+            {
+                bool capped;
+                n = groupWalk<true>(A, inRange, ~v, stk[g], 0u, 0u, A.nNodes, capped);
+                if (capped && o == 0) atomicAdd(&A.ovf[1], 1);
+            }
+            if (o == 0) toff[t] = n;
+        }
+        __syncthreads();
+        const uint32_t total = blockScan(toff, nTok, red);
+        if constexpr (COUNT)
+        {
+            if (threadIdx.x == 0) A.jCount[a] = total;
+        }
+        else
+        {
+            const uint32_t base = A.jOffset[a], limit = A.jOffset[a + 1];
+            for (int t = g; t < nTok; t += kGroups)
+            {
+                const int v = tok[cur][t];
+                const uint32_t at = base + toff[t];
+                if (v >= 0)
+                {
+                    if (o == 0)
+                    {
+                        if (at < limit) A.jList[at] = v;
+                        else            atomicAdd(&A.ovf[0], 1);
+                    }
+                }
+                else
+                {
+                    bool capped;
+                    (void)groupWalk<false>(A, inRange, ~v, stk[g], at, limit, A.nNodes, capped);
+                }
+            }
+        }
+        __syncthreads();   // the block's LDS is reused by the next list
     }
 }
 
@@ -286,12 +503,19 @@ void hmaxUpsweep(State& s)
     }
 }
 
+constexpr int kHeavyBlocks = 1024;
+
+// The walks of one form (count, or fill at jOffset), then the block walks of the long lists.
 template<bool SYM, bool COUNT>
 static void launchList(const ListArgs& a, bool periodic, hipStream_t st)
 {
     const int grid = a.ctl ? std::min(iceil(a.nLeaves, kGroups), 8192) : iceil(a.nLeaves, kGroups);
     if (periodic) jlistKernel<SYM, COUNT, true><<<grid, kListThreads, 0, st>>>(a);
     else          jlistKernel<SYM, COUNT, false><<<grid, kListThreads, 0, st>>>(a);
+    LAUNCH_CHECK();
+    const int hgrid = std::min(a.nLeaves, kHeavyBlocks);
+    if (periodic) jlistHeavyKernel<SYM, COUNT, true><<<hgrid, kListThreads, 0, st>>>(a);
+    else          jlistHeavyKernel<SYM, COUNT, false><<<hgrid, kListThreads, 0, st>>>(a);
     LAUNCH_CHECK();
 }
 
@@ -313,6 +537,12 @@ static ListArgs listArgs(State& s, const int* leaves, int nLeaves, bool symmetri
     a.boxes = s.box.as<DBox>();
     a.ovf = s.ovf.get<int>(4);
     a.nNodes = s.nNodes;
+    a.nodeLeaves = s.nodeLeaves.as<int>();
+    a.heavyN = s.heavyN.get<int>(4);
+    a.heavyList = s.heavyList.get<int>((size_t)s.nLeaves + 1);
+    a.heavyFlag = s.heavyFlag.get<uint8_t>((size_t)s.nLeaves + 1);
+    a.heavySteps = s.listHeavySteps > 0 ? s.listHeavySteps : kHeavySteps;
+    a.expandLeaves = s.listExpandLeaves >= 0 ? s.listExpandLeaves : kExpandLeaves;
     return a;
 }
 
@@ -326,8 +556,15 @@ void launchGatherListsDevice(State& s, const int* leaves, int nCap, uint32_t* jC
     a.jOffset = jOffset;
     a.jList = jList;
     const bool per = s.hBox.size() && s.hBox[0].periodic;
-    if (jList) launchList<false, false>(a, per, s.stream);
-    else       launchList<false, true>(a, per, s.stream);
+    if (jList)
+    {
+        launchList<false, false>(a, per, s.stream);
+    }
+    else
+    {
+        fillU32(reinterpret_cast<uint32_t*>(a.heavyN), 0u, 1, s.stream);
+        launchList<false, true>(a, per, s.stream);
+    }
 }
 
 long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric, bool tight)
@@ -338,6 +575,7 @@ long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric, b
     ListArgs a = listArgs(s, leaves, nLeaves, symmetric, tight);
     a.jCount = jc;
     const bool per = s.hBox.size() && s.hBox[0].periodic;
+    fillU32(reinterpret_cast<uint32_t*>(a.heavyN), 0u, 1, st);
     if (nLeaves > 0)
     {
         if (symmetric) launchList<true, true>(a, per, st);

@@ -287,7 +287,7 @@ std::vector<unsigned long long> planKey(const State& s, const Plan& P, int K)
     add(&P, sizeof(Plan));
     for (const DevBuf* b : {&s.layout, &s.leafNode, &s.leafSet, &s.intFirst, &s.nodeChild, &s.nodeCx, &s.nodeCy,
                             &s.nodeCz, &s.nodeSx, &s.nodeSy, &s.nodeSz, &s.hmaxLeaf, &s.hRemLeaf, &s.box, &s.h,
-                            &s.ovf, &s.scanTmp})
+                            &s.ovf, &s.scanTmp, &s.nodeLeaves, &s.heavyN, &s.heavyList, &s.heavyFlag})
         k.push_back((unsigned long long)(uintptr_t)b->p);
     k.push_back((unsigned long long)s.nSets);
     k.push_back((unsigned long long)s.nNodes);
@@ -402,6 +402,9 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
     s.ovf.get<int>(4);
     s.hmaxLeaf.get<double>((size_t)L + 1);
     s.hRemLeaf.get<double>((size_t)L + 1);
+    s.heavyN.get<int>(4);   // the list walks take these in the captured passes: sized here
+    s.heavyList.get<int>((size_t)L + 1);
+    s.heavyFlag.get<uint8_t>((size_t)L + 1);
     if (s.loopBlocks <= 0 || s.loopBlocksPrecision != s.precision || s.loopBlocksPeriodic != (int)in.periodic)
     {
         s.loopBlocks = densityPersistentBlocks(s.precision, in.periodic);

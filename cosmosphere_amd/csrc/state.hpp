@@ -126,6 +126,7 @@ struct State
     DevBuf nodeCx, nodeCy, nodeCz, nodeSx, nodeSy, nodeSz;   // FP64 geometry (SoA)
     DevBuf nodeHmax;     // force upsweep
     DevBuf intNode;      // node id of each internal node (BFS order)
+    DevBuf nodeLeaves;   // int: leaves under each internal node
     std::vector<int> hIntLevelFirst;          // level ranges of internal nodes (all sets)
 
     // ---- NR and lists ----
@@ -135,6 +136,7 @@ struct State
     DevBuf active;       // u8 per particle: 1 iterating, 2 closure pending, 0 done
     DevBuf itemCount, itemCountAlt, itemOff, itemPos, itemPart;   // NR work items
     DevBuf jCount, jOffset, jList;
+    DevBuf heavyList, heavyFlag, heavyN;   // lists walked by a whole block (long walks)
     int    listLeaves = 0;    // number of leaves the CSR lists cover (all-leaf lists)
     int    listTotal  = 0;
     int    listSymmetric = 0; // 1 when the lists are the force (symmetric) lists
@@ -168,6 +170,8 @@ struct State
     int      loopBlockScale = 4;   // persistent density grid: resident blocks x scale / 4
     int      loopBlocks = 0, loopBlocksPrecision = -1, loopBlocksPeriodic = -1;
     long long loopListCap = 0;     // verification: starting list capacity of the device-driven loop (0: auto)
+    int listHeavySteps = 0;        // verification: walk steps before a list goes to the block walk (0: default)
+    int listExpandLeaves = -1;     // verification: block walk expands nodes above this many leaves (-1: default)
     std::vector<PassRecord> passLog;   // one record per NR pass of the last solve
     int      closureOn = -1;   // -1: on for mixed, off for plain
     int      treeOnly = 0;     // verification: stop after the tree build

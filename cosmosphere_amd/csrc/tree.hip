@@ -401,6 +401,34 @@ __global__ __launch_bounds__(256) void linkKernel(TreeOut t, int nInt, const uin
     writeGeometry(t, child, cs, cl, boxes[b]);
 }
 
+// Leaves under each internal node: a set's leaves are numbered in start-key order, so the
+// leaves of the node's key range [start, start + span) are those between two lower bounds.
+__global__ __launch_bounds__(256) void nodeLeavesKernel(int nInt, const uint64_t* __restrict__ intStart,
+                                                        const int* __restrict__ intLevel,
+                                                        const int* __restrict__ intSet,
+                                                        const int* __restrict__ intNode,
+                                                        const uint64_t* __restrict__ leafStart,
+                                                        const int* __restrict__ leafFirst,
+                                                        const int* __restrict__ intN, int* __restrict__ nodeLeaves)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nInt) return;
+    const int b = intSet[k];
+    const int l0 = leafFirst[b], nl = 7 * intN[b] + 1;
+    const uint64_t start = intStart[k], end = start + levelSpan(intLevel[k]);
+    auto lower = [&](uint64_t key) {
+        int lo = l0, hi = l0 + nl;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi) >> 1;
+            if (leafStart[mid] < key) lo = mid + 1;
+            else                      hi = mid;
+        }
+        return lo;
+    };
+    nodeLeaves[intNode[k]] = lower(end) - lower(start);
+}
+
 __global__ __launch_bounds__(256) void layoutKernel(const uint64_t* __restrict__ keys,
                                                     const uint64_t* __restrict__ leafStart,
                                                     const int* __restrict__ leafSet, int nLeaves,
@@ -706,6 +734,13 @@ void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch
     {
         linkKernel<<<iceil(8LL * nInt, 256), 256, 0, st>>>(t, nInt, intStart, intLevel, intSet, preStart, dIntFirst,
                                                             dIntN, dLeafFirst, dNodeFirst, s.box.as<DBox>());
+        LAUNCH_CHECK();
+    }
+    int* nodeLeaves = s.nodeLeaves.get<int>(M + 1);
+    if (nInt > 0)
+    {
+        nodeLeavesKernel<<<iceil(nInt, 256), 256, 0, st>>>(nInt, intStart, intLevel, intSet, t.intNode, t.leafStart,
+                                                            dLeafFirst, dIntN, nodeLeaves);
         LAUNCH_CHECK();
     }
     uint32_t* layout = s.layout.get<uint32_t>(L + 2);

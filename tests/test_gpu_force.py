@@ -34,8 +34,18 @@ def thermo(c, h, u, gamma=5.0 / 3.0):
     return np.stack([(gamma - 1) * u / rhoh, np.sqrt(gamma * (gamma - 1) * u), np.ones(n), u])
 
 
-@pytest.mark.parametrize("variant", [dict(), dict(disc=1), dict(pdv=0), dict(shock=0, quintic=False)])
-def test_force_parity(variant):
+@pytest.fixture
+def _default_walk():
+    yield
+    for kernel in ("cubic", "quintic"):
+        lib = Library(kernel)
+        lib.set_option("list_heavy_steps", 0)
+        lib.set_option("list_expand_leaves", -1)
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(disc=1), dict(pdv=0), dict(shock=0, quintic=False),
+                                     dict(block=1)])
+def test_force_parity(variant, _default_walk):
     quintic = variant.get("quintic", True)
     c, v, u = setup(quintic=quintic)
     n = c["n"]
@@ -44,6 +54,9 @@ def test_force_parity(variant):
     lib.set_option("precision", 0)
     lib.set_option("max_iter", 100)
     lib.set_option("tree_only", 0)
+    if variant.get("block"):   # every list (gather and symmetric) through the block walk
+        lib.set_option("list_heavy_steps", 1)
+        lib.set_option("list_expand_leaves", 0)
     out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], hfact=c["hfact"], v=v)
     th = thermo(c, out["h"], u)
     lib.upload_sorted(THERMO, th)

@@ -19,7 +19,8 @@ from gpu_common import DEFAULT_ITEMS, dataset, gpu_run, oracle_run, rel_err
 pytestmark = pytest.mark.gpu
 
 MAPPINGS = [("whole", {"mapping": 1}), ("flat", {"mapping": 2}), ("auto4", {"sparse": 4}),
-            ("auto64", {"sparse": 64}), ("nocull", {"cull": 0}), ("nocull_whole", {"cull": 0, "mapping": 1})]
+            ("auto64", {"sparse": 64}), ("nocull", {"cull": 0}), ("nocull_whole", {"cull": 0, "mapping": 1}),
+            ("block_walk", {"list_heavy_steps": 1, "list_expand_leaves": 0})]
 
 
 @pytest.fixture(autouse=True)

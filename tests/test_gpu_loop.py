@@ -46,7 +46,7 @@ def same(a, b):
 VARIANTS = [("graph_k4", {}), ("graph_k1", {"graph_passes": 1}), ("graph_k3", {"graph_passes": 3}),
             ("direct", {"graph": 0}), ("no_lpt", {"lpt": 0}), ("overflow", {"loop_list_cap": 64}),
             ("flat", {"mapping": 2}), ("whole", {"mapping": 1}), ("nocull", {"cull": 0}),
-            ("nounits", {"units": 0})]
+            ("nounits", {"units": 0}), ("block_walk", {"list_heavy_steps": 1, "list_expand_leaves": 0})]
 
 
 @pytest.mark.parametrize("label,opts", VARIANTS, ids=[v[0] for v in VARIANTS])
@@ -56,6 +56,8 @@ def test_device_loop_bit_identical(name, precision, label, opts):
     c = dataset(name)
     print("case", name, precision, label, flush=True)
     base_opts = {k: v for k, v in opts.items() if k in ("mapping", "cull", "units")}
+    if label == "block_walk":
+        base_opts = {}   # the block walk's lists are the group walk's: the host loop keeps the default
     host = solve(c, precision, loop=0, **base_opts)
     dev = solve(c, precision, loop=1, **opts)
     same(dev, host)

@@ -10,7 +10,7 @@ the oracle's closure at the GPU's own h (the mixed closure sums FP32 pair terms 
 import numpy as np
 import pytest
 
-from gpu_common import dataset, gpu_run, oracle_run, rel_err
+from gpu_common import FORCE_BLOCK_WALK, dataset, gpu_run, oracle_run, rel_err
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -70,11 +70,13 @@ def test_density_plain_fp64(name):
     assert abs(lib.stat("pairs") - sum(o.get("pairsPerIter"))) <= 1e-6 * sum(o.get("pairsPerIter")) + 2
 
 
+@pytest.mark.parametrize("walk", ["group", "block"])
 @pytest.mark.parametrize("name", ["lattice", "plummer_quintic"])
-def test_jlists_bit_exact(name):
-    """All-leaf gather lists at the converged h (left by the gradient sweep)."""
+def test_jlists_bit_exact(name, walk):
+    """All-leaf gather lists at the converged h (left by the gradient sweep), walked by groups of
+    8 lanes or (block) every list handed to the block walk, which splits it into subtree walks."""
     c = dataset(name)
-    lib, _ = gpu_run(c, precision=0, gradients=True)
+    lib, _ = gpu_run(c, precision=0, gradients=True, **(FORCE_BLOCK_WALK if walk == "block" else {}))
     o, _ = oracle_run(c, gradients=False)
     o.set_h_sorted(lib.array("h"))
     o.hmax_all()
