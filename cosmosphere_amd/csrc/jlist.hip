@@ -133,8 +133,9 @@ constexpr int kStack       = 152;                // >= 7*20 + 8
 // buffer holds the result; the groups then walk the out(C) tokens at once, count them, scan
 // the token sizes and fill each token at its offset.  The list is the sequential walk's,
 // entry for entry.
-constexpr int kHeavySteps   = 512;    // per 32768 lists of the launch (a launch of many lists
-                                      // keeps the chip busy through long walks)
+constexpr int kHeavySteps   = 256;    // per 32768 lists of the launch (a launch of many lists
+                                      // keeps the chip busy through long walks); 128, 512 and
+                                      // 1024 were slower on the 8M Plummer mixture
 constexpr int kTokCap       = 2048;
 constexpr int kExpandLeaves = 256;
 
