@@ -169,7 +169,7 @@ class Library:
 
     PASS_FIELDS = ("leaves", "items", "iterating", "candidates", "pairs", "closure_candidates", "kernel_s",
                    "per_particle", "list_s", "compact_s", "list_entries", "max_list_log2",
-                   "heavy_items")
+                   "heavy_items", "split_items")
 
     def passes(self):
         """One row per NR pass of the last solve (columns PASS_FIELDS)."""

@@ -343,6 +343,7 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "loop_blocks_scale") s.loopBlockScale = std::max(1, (int)value);
     else if (nm == "loop_list_cap") s.loopListCap = (long long)value;
     else if (nm == "list_heavy_steps") s.listHeavySteps = (int)value;
+    else if (nm == "split_len") s.splitLen = (int)value;
     else if (nm == "list_expand_leaves") s.listExpandLeaves = (int)value;
     else
     {

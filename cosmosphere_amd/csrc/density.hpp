@@ -26,7 +26,9 @@ struct PassCtl
     uint32_t perParticle;   // the current pass uses per-particle items for sparse leaves
     uint32_t isNR;          // the current pass holds NR evaluations (else closures only)
     uint32_t queueHead;     // next position of the work queue
-    uint32_t qBucket[32];   // work queue: [0] largest cost bucket, [1] heavy items at its head
+    uint32_t qBucket[32];   // work queue: [0] largest cost bucket, [1] entries at its head (heavy items, split)
+    uint32_t nSplit;        // heavy items split over several wavefronts this pass (slots taken)
+    uint32_t qExtra;        // queue-head entries beyond one per heavy item
     unsigned long long nIterating;      // particles in state 1 at the start of the pass
     unsigned long long listNeed;        // list entries of the current pass
     unsigned long long tBegin;          // wall clock at the start of the pass
@@ -37,7 +39,7 @@ struct PassCtl
 struct PassLogDev
 {
     unsigned long long nAL, nItems, nIterating, cand, pairs, closCand, closPairs;
-    unsigned long long tBegin, tKStart, tKEnd, tEnd, perParticle, listTotal, isNR, maxListLog2, heavyItems;
+    unsigned long long tBegin, tKStart, tKEnd, tEnd, perParticle, listTotal, isNR, maxListLog2, heavyItems, splitItems;
 };
 
 struct DensArgs
@@ -70,7 +72,17 @@ struct DensArgs
     PassCtl*        ctl;        // device-driven pass: persistent waves pull items from a queue
     const int*      qOrder;     // ... heavy items qOrder[0, ctl->qBucket[1]) first, then every item
     const uint8_t*  qFlag;      //     not flagged heavy, in item order (nullptr: item order only)
+    const uint32_t* qSub;       // ... per head entry: sub-item | count << 8 | slot << 16 (nullptr: none split)
+    double*         splitPart;  // split items: per slot, sub-item and particle, the two partial sums
+    uint32_t*       splitDone;  // ... per slot, sub-items finished (the last one resets it)
 };
+
+// Split work items: a heavy item with a list of more than kSplitLen entries is cut into at
+// most kMaxSub sub-items (list ranges) run by different wavefronts; the last to finish adds
+// the partial sums in sub-item order and ends the particles.  At most kMaxSplit per pass.
+constexpr int kSplitLen = 2048;
+constexpr int kMaxSub   = 32;
+constexpr int kMaxSplit = 256;
 
 // Device work counters (u64, summed over the counter slots).
 enum Counter

@@ -57,6 +57,8 @@ __global__ void passBeginKernel(PassCtl* __restrict__ C, int maxIter, int sparse
     if (t < kCostBuckets) C->qBucket[t] = 0u;
     if (t != 0) return;
     C->queueHead = 0u;
+    C->nSplit = 0u;
+    C->qExtra = 0u;
     C->tStart = ~0ull;
     C->tEnd = 0ull;
     C->tBegin = clockNow();
@@ -105,6 +107,7 @@ __device__ inline int costBucket(const PassCtl* C, const int* __restrict__ itemP
 constexpr int kCostThreads = 256;
 constexpr int kHeavySpan   = 2;   // heavy: a list at least 1/4 as long as the longest (in buckets)
 constexpr int kHeavyMean   = 8;   // ... and at least 8 times the mean list length of the pass
+constexpr int kFewItems    = 4096;   // a pass of at most this many items leaves most of the chip idle
 
 // Largest cost bucket of the pass's items -> qBucket[0].
 __global__ __launch_bounds__(kCostThreads) void costMaxKernel(PassCtl* __restrict__ C, const int* __restrict__ itemPos,
@@ -123,10 +126,17 @@ __global__ __launch_bounds__(kCostThreads) void costMaxKernel(PassCtl* __restric
 // are few, and started last they would be the pass's tail), then every item in its own order
 // (the leaves' Hilbert order, which keeps the j-particles that neighbouring items share in L2),
 // skipping the heavy ones (qFlag).  A workload whose lists are all alike has no heavy item.
+//
+// A heavy item whose list exceeds splitLen entries (a whole leaf of at most 64 particles, or one
+// particle) becomes up to kMaxSub sub-items at the head, while slots and head room last.
 __global__ __launch_bounds__(kCostThreads) void costHeavyKernel(PassCtl* __restrict__ C,
                                                                  const int* __restrict__ itemPos,
                                                                  const uint32_t* __restrict__ jCount,
-                                                                 int* __restrict__ qOrder, uint8_t* __restrict__ qFlag)
+                                                                 int* __restrict__ qOrder, uint8_t* __restrict__ qFlag,
+                                                                 uint32_t* __restrict__ qSub,
+                                                                 const int* __restrict__ leaves,
+                                                                 const uint32_t* __restrict__ layout,
+                                                                 const int* __restrict__ itemPart, int splitLen)
 {
     if (C->skip) return;
     const int n = (int)C->nItems;
@@ -135,10 +145,37 @@ __global__ __launch_bounds__(kCostThreads) void costHeavyKernel(PassCtl* __restr
     for (int k = blockIdx.x * kCostThreads + threadIdx.x; k < n; k += gridDim.x * kCostThreads)
     {
         const int a = C->perParticle ? itemPos[k] : k;
+        // heavy: among the longest lists and well above the mean; in a pass of few items (too
+        // few to fill the chip) every list long enough to split
         const bool heavy =
-            costBucket(C, itemPos, jCount, k) >= cut && (unsigned long long)jCount[a] >= kHeavyMean * mean;
+            (costBucket(C, itemPos, jCount, k) >= cut && (unsigned long long)jCount[a] >= kHeavyMean * mean) ||
+            (qSub && splitLen > 0 && n <= kFewItems && jCount[a] > (uint32_t)splitLen);
         qFlag[k] = heavy ? 1 : 0;
-        if (heavy) qOrder[atomicAdd(&C->qBucket[1], 1u)] = k;
+        if (!heavy) continue;
+        uint32_t nSub = 1u, slot = 0u;
+        const uint32_t len = jCount[a];
+        if (qSub && splitLen > 0 && len > (uint32_t)splitLen)
+        {
+            const int l = leaves[a];
+            const bool onePart = C->perParticle && itemPart[k] >= 0;   // a whole leaf: one batch only
+            if (onePart || layout[l + 1] - layout[l] <= (uint32_t)kWave)
+            {
+                const uint32_t want = min((len + (uint32_t)splitLen - 1u) / (uint32_t)splitLen, (uint32_t)kMaxSub);
+                const uint32_t sl = atomicAdd(&C->nSplit, 1u);
+                const uint32_t ex = atomicAdd(&C->qExtra, want - 1u);
+                if (sl < (uint32_t)kMaxSplit && ex + want - 1u <= (uint32_t)(kMaxSplit * kMaxSub))
+                {
+                    nSub = want;
+                    slot = sl;
+                }
+            }
+        }
+        const uint32_t base = atomicAdd(&C->qBucket[1], nSub);
+        for (uint32_t s = 0; s < nSub; ++s)
+        {
+            qOrder[base + s] = k;
+            if (qSub) qSub[base + s] = s | (nSub << 8) | (slot << 16);
+        }
     }
 }
 
@@ -182,6 +219,7 @@ __global__ __launch_bounds__(256) void passEndKernel(PassCtl* __restrict__ C,
         r.isNR = C->isNR;
         r.maxListLog2 = C->qBucket[0];
         r.heavyItems = C->qBucket[1];
+        r.splitItems = min(C->nSplit, (uint32_t)kMaxSplit);
     }
     C->passes += 1u;
     if (C->isNR) C->iters += 1u;
@@ -225,6 +263,10 @@ struct Plan
     int*        itemPart;
     int*        qOrder;
     uint8_t*    qFlag;
+    uint32_t*   qSub;
+    double*     splitPart;
+    uint32_t*   splitDone;
+    int         splitLen;
     uint32_t*   jc;
     uint32_t*   jo;
     int*        jList;
@@ -261,7 +303,8 @@ void enqueuePass(State& s, const Plan& P)
     if (P.lpt)
     {
         costMaxKernel<<<P.costBlocks, kCostThreads, 0, st>>>(P.ctl, P.itemPos, P.jc);
-        costHeavyKernel<<<P.costBlocks, kCostThreads, 0, st>>>(P.ctl, P.itemPos, P.jc, P.qOrder, P.qFlag);
+        costHeavyKernel<<<P.costBlocks, kCostThreads, 0, st>>>(P.ctl, P.itemPos, P.jc, P.qOrder, P.qFlag, P.qSub,
+                                                               P.leaves, layout, P.itemPart, P.splitLen);
         LAUNCH_CHECK();
     }
     launchDensityPersistent(P.da, P.precision, P.periodic, P.blocks, st);
@@ -393,8 +436,15 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
     P.logCap = s.maxIter + 4;
     P.ctl = s.passCtl.get<PassCtl>(1);
     P.log = s.passLogDev.get<PassLogDev>((size_t)P.logCap);
-    P.qOrder = s.qOrder.get<int>((size_t)std::max(na, L) + 1);
-    P.qFlag = s.qFlag.get<uint8_t>((size_t)std::max(na, L) + 1);
+    // queue head: a heavy item per item at most, plus the sub-items of the split ones
+    const size_t qCap = (size_t)std::max(na, L) + 1 + (size_t)kMaxSplit * kMaxSub;
+    P.qOrder = s.qOrder.get<int>(qCap);
+    P.qFlag = s.qFlag.get<uint8_t>(qCap);
+    P.splitLen = s.splitLen >= 0 ? s.splitLen : kSplitLen;
+    P.qSub = P.splitLen > 0 ? s.qSub.get<uint32_t>(qCap) : nullptr;
+    P.splitPart = s.splitPart.get<double>((size_t)kMaxSplit * kMaxSub * kWave * 2);
+    P.splitDone = s.splitDone.get<uint32_t>((size_t)kMaxSplit);
+    fillU32(P.splitDone, 0u, (size_t)kMaxSplit, st);   // outside the graph: every pass leaves them at 0
     P.jc = s.jCount.get<uint32_t>((size_t)L + 1);
     P.jo = s.jOffset.get<uint32_t>((size_t)L + 2);
     P.counters = s.counters.get<unsigned long long>((size_t)kCounterSlots * kCounterStride + 64);
@@ -435,6 +485,9 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
     da.ctl = P.ctl;
     da.qOrder = P.lpt ? P.qOrder : nullptr;
     da.qFlag = P.lpt ? P.qFlag : nullptr;
+    da.qSub = P.lpt ? P.qSub : nullptr;
+    da.splitPart = P.splitPart;
+    da.splitDone = P.splitDone;
     da.itemPos = P.itemPos;
     da.itemPart = P.itemPart;
     da.jOffset = P.jo;
@@ -537,7 +590,8 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
         const double tc = ran ? (double)(r.tEnd - r.tKEnd) / hz : (double)(r.tEnd - r.tBegin) / hz;
         s.passLog.push_back(PassRecord{(double)r.nAL, (double)r.nItems, (double)r.nIterating, (double)r.cand,
                                        (double)r.pairs, (double)r.closCand, tk, (double)r.perParticle, tj, tc,
-                                       (double)r.listTotal, (double)r.maxListLog2, (double)r.heavyItems});
+                                       (double)r.listTotal, (double)r.maxListLog2, (double)r.heavyItems,
+                                       (double)r.splitItems});
         if (r.isNR)
         {
             t.jbuild += tj;

@@ -282,6 +282,40 @@ constexpr UnitShapes makeUnitShapes()
 
 __constant__ constexpr UnitShapes kUnitShapes = makeUnitShapes();
 
+// The end of one sub-item of a split work item: particle r's sums (lanes with r >= 0) go to the
+// item's slot; the wavefront whose sub-item completes the item (the count of finished sub-items
+// reaches the item's count) adds every sub-item's partials in sub-item order and returns true
+// with the totals in (s0, s1), after which it ends the particles.  The release fence before
+// the count and the acquire fence after it order the partials across the chip's L2 caches.
+__device__ inline bool splitCombine(const DensArgs& A, uint32_t subInfo, int r, double& s0, double& s1)
+{
+    const uint32_t sub = subInfo & 0xFFu, nSub = (subInfo >> 8) & 0xFFu, slot = subInfo >> 16;
+    double* P = A.splitPart + (size_t)slot * kMaxSub * kWave * 2;
+    if (r >= 0)
+    {
+        P[((size_t)sub * kWave + r) * 2] = s0;
+        P[((size_t)sub * kWave + r) * 2 + 1] = s1;
+    }
+    __threadfence();
+    uint32_t done = 0;
+    if (laneId() == 0) done = atomicAdd(&A.splitDone[slot], 1u);
+    done = __builtin_amdgcn_readfirstlane(done);
+    if (done != nSub - 1u) return false;
+    __threadfence();
+    if (r >= 0)
+    {
+        s0 = 0.0;
+        s1 = 0.0;
+        for (uint32_t q = 0; q < nSub; ++q)
+        {
+            s0 += P[((size_t)q * kWave + r) * 2];
+            s1 += P[((size_t)q * kWave + r) * 2 + 1];
+        }
+    }
+    if (laneId() == 0) A.splitDone[slot] = 0u;   // the slot is free for the next pass
+    return true;
+}
+
 // Particle sums from the accumulators: (sum W, sum -q dW/dq - 3W).
 template<bool MIXED>
 __device__ inline void particleSums(double s0, double s1, double& wsum, double& gsum)
@@ -326,11 +360,20 @@ void densityKernel(DensArgs A)
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = laneId();
     unsigned long long ctr[4] = {0, 0, 0, 0};   // NR pairs, NR candidates, closure pairs, closure candidates
 
-    auto run = [&](const int item, unsigned long long (&cc)[4]) {
+    // subInfo: 0, or (sub-item, count, slot) of a split item: list range sub of count, partial sums
+    auto run = [&](const int item, unsigned long long (&cc)[4], const uint32_t subInfo) {
         const int a = PP ? A.itemPos[item] : item;
         const int part = PP ? __builtin_amdgcn_readfirstlane(A.itemPart[item]) : -1;
         const int iLeaf = A.leaves ? A.leaves[a] : a;
-        const uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
+        const uint32_t nSub = (subInfo >> 8) & 0xFFu;
+        uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
+        if (nSub > 1u)
+        {
+            const uint32_t len = hi - lo, chunk = (len + nSub - 1u) / nSub, sub = subInfo & 0xFFu;
+            const uint32_t b = lo;
+            lo = b + min(len, sub * chunk);
+            hi = b + min(len, (sub + 1u) * chunk);
+        }
         const DBox box = A.boxes[A.leafSet[iLeaf]];
 
         if (PP && part >= 0)
@@ -368,6 +411,7 @@ void densityKernel(DensArgs A)
             const int slot = isClosureEval<MODE>(st) ? 2 : 0;
             cc[slot] += (unsigned long long)acc;
             cc[slot + 1] += cand;
+            if (nSub > 1u && !splitCombine(A, subInfo, lane == 0 ? 0 : -1, s0, s1)) return;
             double wsum, gsum;
             particleSums<kMixed>(s0, s1, wsum, gsum);
             if (lane == 0) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
@@ -553,6 +597,7 @@ void densityKernel(DensArgs A)
                         waveLdsSync();
                         tq[w][kWave + lane] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);   // the padding back
                         waveLdsSync();
+                        if (nSub > 1u && !splitCombine(A, subInfo, own ? lane : -1, s0, s1)) continue;
                         double wsum, gsum;
                         particleSums<kMixed>(s0, s1, wsum, gsum);
                         if (own) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
@@ -620,6 +665,7 @@ void densityKernel(DensArgs A)
                         const double t0 = __shfl_down(s0, off, kWave), t1 = __shfl_down(s1, off, kWave);
                         if (sub + off < g) { s0 += t0; s1 += t1; }
                     }
+                    if (nSub > 1u && !splitCombine(A, subInfo, (has && sub == 0) ? grp : -1, s0, s1)) continue;
                     double wsum, gsum;
                     particleSums<kMixed>(s0, s1, wsum, gsum);
                     if (has && sub == 0) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
@@ -644,19 +690,19 @@ void densityKernel(DensArgs A)
             k = __builtin_amdgcn_readfirstlane(k);
             if (k < nHeavy)
             {
-                run(A.qOrder[k], ctr);
+                run(A.qOrder[k], ctr, A.qSub ? A.qSub[k] : 0u);
                 continue;
             }
             k -= nHeavy;
             if (k >= nItems) break;
             if (nHeavy && A.qFlag[k]) continue;   // taken from the head already
-            run(k, ctr);
+            run(k, ctr, 0u);
         }
     }
     else
     {
         const int item = blockIdx.x * kNrWaves + w;
-        if (item < A.nItems) run(item, ctr);
+        if (item < A.nItems) run(item, ctr, 0u);
     }
     blockCounters(A.counters, ctr);
     if (QUEUE && threadIdx.x == 0) atomicMax(&A.ctl->tEnd, (unsigned long long)wall_clock64());

@@ -252,7 +252,7 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         const double tj = secs(a0, a1), tk = secs(a1, a2), tc = secs(a2, a3);
         s.passLog.push_back(PassRecord{(double)nALpass, (double)da.nItems, (double)nIterating, (double)c[kCtrCand],
                                        (double)c[kCtrPairs], (double)c[kCtrClosCand], tk, (double)perParticle,
-                                       tj, tc, (double)s.listTotal, 0.0, 0.0});
+                                       tj, tc, (double)s.listTotal, 0.0, 0.0, 0.0});
         if (nrPass)
         {
             t.jbuild += tj;

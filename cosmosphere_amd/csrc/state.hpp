@@ -63,7 +63,8 @@ struct PassRecord
     double compactSeconds;   // kernel end to pass end: compaction
     double listEntries;      // entries of the pass's lists
     double maxListLog2;      // device-driven passes: floor(log2) of the longest list (with the cost order on)
-    double heavyItems;       // device-driven passes: items taken first by the cost order
+    double heavyItems;       // device-driven passes: queue-head entries (heavy items and their sub-items)
+    double splitItems;       // device-driven passes: heavy items split over several wavefronts
 };
 
 struct ForceTimings
@@ -137,6 +138,7 @@ struct State
     DevBuf itemCount, itemCountAlt, itemOff, itemPos, itemPart;   // NR work items
     DevBuf jCount, jOffset, jList;
     DevBuf heavyList, heavyFlag, heavyN;   // lists walked by a whole block (long walks)
+    DevBuf qSub, splitPart, splitDone;     // device-driven loop: heavy items split over wavefronts
     int    listLeaves = 0;    // number of leaves the CSR lists cover (all-leaf lists)
     int    listTotal  = 0;
     int    listSymmetric = 0; // 1 when the lists are the force (symmetric) lists
@@ -172,6 +174,7 @@ struct State
     long long loopListCap = 0;     // verification: starting list capacity of the device-driven loop (0: auto)
     int listHeavySteps = 0;        // verification: walk steps before a list goes to the block walk (0: default)
     int listExpandLeaves = -1;     // verification: block walk expands nodes above this many leaves (-1: default)
+    int splitLen = -1;             // device-driven loop: heavy items with longer lists are split (-1: default, 0: off)
     std::vector<PassRecord> passLog;   // one record per NR pass of the last solve
     int      closureOn = -1;   // -1: on for mixed, off for plain
     int      treeOnly = 0;     // verification: stop after the tree build

@@ -106,7 +106,9 @@ int cosmo_ext_solve_dens_h(int n, double pmass, int mode, int withGradients, int
  * "tree_only" (1: stop after the tree build, for verification of the tree stages),
  * "list_heavy_steps" and "list_expand_leaves" (verification of the block walk of long lists:
  * walk steps per 32768 lists before a list is handed to a whole block, default 256; nodes with
- * more leaves than this are expanded into separate walks, default 256).
+ * more leaves than this are expanded into separate walks, default 256), "split_len" (device-driven
+ * loop: a heavy work item whose list has more entries is split over several wavefronts, default
+ * 2048; 0: never).
  * The environment variable COSMO_PRECISION=mixed|fp64 sets the precision for
  * densityiterate_gpu_c. */
 void cosmo_ext_set_option(const char* name, double value);
