@@ -125,10 +125,8 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
         double rhoi = 0, gradhi = 0, divs = 0, dv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, da[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         double rxx = 0, rxy = 0, rxz = 0, ryy = 0, ryz = 0, rzz = 0;
         auto keep = [&](int jl) {
-            const int node = A.leafNode[jl];
             for (int k = 0; k < c; ++k)
-                if (pointNodeGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.cx, A.cy, A.cz, A.sx, A.sy, A.sz, node,
-                                       box) <= cp[w][3][k])
+                if (pointLeafGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.lbox, A.lboxN, jl, box) <= cp[w][3][k])
                     return true;
             return false;
         };

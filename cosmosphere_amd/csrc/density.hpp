@@ -56,6 +56,8 @@ struct DensArgs
     const int*      leafSet;
     const double*   cx; const double* cy; const double* cz;
     const double*   sx; const double* sy; const double* sz;
+    const double*   lbox;       // tight boxes of the leaves' particles (culling), lboxN leaves
+    int             lboxN;
     const double*   hmaxLeaf;
     const double*   x; const double* y; const double* z;
     double*         h;
@@ -112,6 +114,8 @@ struct GradArgs
     const DBox*     boxes;
     double          pmass;
     const int*      leafNode;                                 // per-particle culling of list entries
+    const double*   lbox;                                     // ... against the leaves' tight boxes
+    int             lboxN;
     const double*   cx; const double* cy; const double* cz;
     const double*   sx; const double* sy; const double* sz;
 };
@@ -135,6 +139,8 @@ struct ForceArgs
     int             pdv, shock;
     unsigned long long* counters;
     const int*      leafNode;                                 // per-particle culling of list entries
+    const double*   lbox;                                     // ... against the leaves' tight boxes
+    int             lboxN;
     const double*   cx; const double* cy; const double* cz;
     const double*   sx; const double* sy; const double* sz;
     const double*   hmaxLeaf;                                 // reach of a j-leaf's own supports

@@ -98,14 +98,13 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
             int acc = 0;
             uint32_t cand = 0;
             auto keep = [&](int jl) {
-                const int node = A.leafNode[jl];
                 double hj = A.hmaxLeaf[jl];
                 const double rj = (isfinite(hj) && hj > 0.0) ? kRad * hj : 0.0;
                 for (int k = 0; k < c; ++k)
                 {
                     const double r = fmax(cp[w][3][k], rj);
-                    if (pointNodeGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.cx, A.cy, A.cz, A.sx, A.sy, A.sz,
-                                           node, box) <= r * r * kCullSlack)
+                    if (pointLeafGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.lbox, A.lboxN, jl, box) <=
+                        r * r * kCullSlack)
                         return true;
                 }
                 return false;
@@ -296,14 +295,13 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixed(ForceArgs A)
             int acc = 0;
             uint32_t cand = 0;
             auto keep = [&](int jl) {
-                const int node = A.leafNode[jl];
                 double hj = A.hmaxLeaf[jl];
                 const double rj = (isfinite(hj) && hj > 0.0) ? kRad * hj : 0.0;
                 for (int k = 0; k < c; ++k)
                 {
                     const double r = fmax(cp[w][3][k], rj);
-                    if (pointNodeGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.cx, A.cy, A.cz, A.sx, A.sy, A.sz,
-                                           node, box) <= r * r * kCullSlack)
+                    if (pointLeafGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.lbox, A.lboxN, jl, box) <=
+                        r * r * kCullSlack)
                         return true;
                 }
                 return false;

@@ -145,11 +145,10 @@ __global__ __launch_bounds__(kCostThreads) void costHeavyKernel(PassCtl* __restr
     for (int k = blockIdx.x * kCostThreads + threadIdx.x; k < n; k += gridDim.x * kCostThreads)
     {
         const int a = C->perParticle ? itemPos[k] : k;
-        // heavy: among the longest lists and well above the mean; in a pass of few items (too
-        // few to fill the chip) every list long enough to split
-        const bool heavy =
-            (costBucket(C, itemPos, jCount, k) >= cut && (unsigned long long)jCount[a] >= kHeavyMean * mean) ||
-            (qSub && splitLen > 0 && n <= kFewItems && jCount[a] > (uint32_t)splitLen);
+        // heavy: among the longest lists and, unless the pass has too few items to fill the chip,
+        // well above the mean
+        const bool heavy = costBucket(C, itemPos, jCount, k) >= cut &&
+                           (n <= kFewItems || (unsigned long long)jCount[a] >= kHeavyMean * mean);
         qFlag[k] = heavy ? 1 : 0;
         if (!heavy) continue;
         uint32_t nSub = 1u, slot = 0u;

@@ -19,12 +19,10 @@
  *     to millions of candidates: spreading them over wavefronts shortens the pass.
  *
  * Pruning.  The lists of a FLAT pass are the reference's lists trimmed to the reach of the
- * particles still to evaluate (jlist.hip, tight lists).  A per-particle item further skips
- * the list entries whose leaf box lies beyond R h_i of its particle.  Skipped leaves hold no
- * pair with q < R (the test keeps two finest cells and a 1e-5 relative margin: a particle
- * lies within one finest cell of its node's box, since the key quantisation floor(x m) - lo m
- * truncates up to one cell low), so the accepted pairs, and in the plain form the sums of a
- * whole-leaf item, are unchanged.
+ * particles still to evaluate (jlist.hip, tight lists).  Items further skip the list entries
+ * whose leaf's particles (their tight box) lie beyond R h_i of every particle evaluated (a
+ * 1e-5 relative margin on the squared reach).  Skipped leaves hold no pair with q < R, so the
+ * accepted pairs are unchanged.
  *
  * Per-particle state (u8): 1 = still iterating, 2 = converged, closure pending (rho and
  * d(rho)/d(h) are evaluated once more, at the final h, in the next pass), 0 = done.  With the
@@ -145,27 +143,6 @@ __device__ inline void imageFP64(double& dx, double& dy, double& dz, const DBox&
         if (fabs(dy) > 0.5 * box.l[1]) dy = dy - box.l[1] * copysign(1.0, dy);
         if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
     }
-}
-
-// Squared distance from the box (c, half-size s) to leaf jl's node box, both widened by one
-// finest cell, nearest periodic image per axis.
-template<bool PER>
-__device__ inline double gapToLeaf(const DensArgs& A, const DBox& box, double c0, double c1, double c2, double s0,
-                                   double s1, double s2, int jl)
-{
-    const int node = A.leafNode[jl];
-    const double c[3] = {c0, c1, c2}, s[3] = {s0, s1, s2};
-    const double cj[3] = {A.cx[node], A.cy[node], A.cz[node]}, sj[3] = {A.sx[node], A.sy[node], A.sz[node]};
-    double d2 = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-    {
-        double dx = cj[a] - c[a];
-        if constexpr (PER) dx -= box.l[a] * rint(dx * box.il[a]);
-        const double v = fmax(fabs(dx) - s[a] - sj[a] - 2.0 * kFinestCell * box.l[a], 0.0);
-        d2 += v * v;
-    }
-    return d2;
 }
 
 // Plain pair term (the reference's arithmetic): sums W and -q dW/dq - 3W when q < R.
@@ -320,6 +297,7 @@ __device__ inline bool splitCombine(const DensArgs& A, uint32_t subInfo, int r, 
 template<bool MIXED>
 __device__ inline void particleSums(double s0, double s1, double& wsum, double& gsum)
 {
+#pragma clang fp contract(off)
     if constexpr (MIXED)
     {
         wsum = kWw * s0;
@@ -389,7 +367,7 @@ void densityKernel(DensArgs A)
             double s0 = 0, s1 = 0;
             int acc = 0;
             uint32_t cand = 0;
-            auto keep = [&](int jl) { return gapToLeaf<PER>(A, box, xi, yi, zi, 0.0, 0.0, 0.0, jl) <= reach2; };
+            auto keep = [&](int jl) { return pointLeafGap2<PER>(xi, yi, zi, A.lbox, A.lboxN, jl, box) <= reach2; };
             auto pair = [&](int, int j) {
                 double dx = xi - A.x[j], dy = yi - A.y[j], dz = zi - A.z[j];
                 if constexpr (kMixed)
@@ -470,10 +448,9 @@ void densityKernel(DensArgs A)
                 auto keep = [&](int jl) {
                     if constexpr (CULL)
                     {
-                        const int jn = A.leafNode[jl];
                         for (int k = 0; k < c; ++k)
-                            if (pointNodeGap2<PER>(cpw[w][0][k], cpw[w][1][k], cpw[w][2][k], A.cx, A.cy, A.cz, A.sx,
-                                                   A.sy, A.sz, jn, box) <= cpw[w][3][k])
+                            if (pointLeafGap2<PER>(cpw[w][0][k], cpw[w][1][k], cpw[w][2][k], A.lbox, A.lboxN, jl, box) <=
+                                cpw[w][3][k])
                                 return true;
                         return false;
                     }
@@ -688,15 +665,22 @@ void densityKernel(DensArgs A)
             int k = 0;
             if (lane == 0) k = (int)atomicAdd(&C->queueHead, 1u);
             k = __builtin_amdgcn_readfirstlane(k);
+            // one call site: an item's arithmetic is the same code wherever the queue puts it
+            int item;
+            uint32_t subInfo = 0u;
             if (k < nHeavy)
             {
-                run(A.qOrder[k], ctr, A.qSub ? A.qSub[k] : 0u);
-                continue;
+                item = A.qOrder[k];
+                if (A.qSub) subInfo = A.qSub[k];
             }
-            k -= nHeavy;
-            if (k >= nItems) break;
-            if (nHeavy && A.qFlag[k]) continue;   // taken from the head already
-            run(k, ctr, 0u);
+            else
+            {
+                k -= nHeavy;
+                if (k >= nItems) break;
+                if (nHeavy && A.qFlag[k]) continue;   // taken from the head already
+                item = k;
+            }
+            run(__builtin_amdgcn_readfirstlane(item), ctr, __builtin_amdgcn_readfirstlane(subInfo));
         }
     }
     else

@@ -113,6 +113,7 @@ static DensArgs densArgs(State& s, const int* leaves, int nLeaves, double pmass,
     a.leafNode = s.leafNode.as<int>();
     a.leafSet = s.leafSet.as<int>();
     a.cx = s.nodeCx.as<double>(); a.cy = s.nodeCy.as<double>(); a.cz = s.nodeCz.as<double>();
+    a.lbox = s.leafBox.as<double>(); a.lboxN = s.nLeaves;
     a.sx = s.nodeSx.as<double>(); a.sy = s.nodeSy.as<double>(); a.sz = s.nodeSz.as<double>();
     a.hmaxLeaf = s.hmaxLeaf.as<double>();
     a.x = s.x.as<double>(); a.y = s.y.as<double>(); a.z = s.z.as<double>();
@@ -348,6 +349,7 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         g.pmass = pmass;
         g.leafNode = s.leafNode.as<int>();
         g.cx = s.nodeCx.as<double>(); g.cy = s.nodeCy.as<double>(); g.cz = s.nodeCz.as<double>();
+        g.lbox = s.leafBox.as<double>(); g.lboxN = s.nLeaves;
         g.sx = s.nodeSx.as<double>(); g.sy = s.nodeSy.as<double>(); g.sz = s.nodeSz.as<double>();
         launchGradients(g, per, st);
         launchDeadGradients(na, n, s.divv.as<double>(), s.xi.as<double>(), s.ddivvdt.as<double>(), st);
@@ -419,6 +421,7 @@ ForceTimings computeForces(State& s, double pmass, double beta, double alphau, b
     a.counters = ctr;
     a.leafNode = s.leafNode.as<int>();
     a.cx = s.nodeCx.as<double>(); a.cy = s.nodeCy.as<double>(); a.cz = s.nodeCz.as<double>();
+    a.lbox = s.leafBox.as<double>(); a.lboxN = s.nLeaves;
     a.sx = s.nodeSx.as<double>(); a.sy = s.nodeSy.as<double>(); a.sz = s.nodeSz.as<double>();
     a.hmaxLeaf = s.hmaxLeaf.as<double>();
     const bool mixed = s.precision == kMixed;

@@ -429,6 +429,43 @@ __global__ __launch_bounds__(256) void nodeLeavesKernel(int nInt, const uint64_t
     nodeLeaves[intNode[k]] = lower(end) - lower(start);
 }
 
+// Tight box of each leaf's particles: centre and half-size per axis, the half-size rounded up
+// by 2^-50 of the coordinates' magnitude so that the box contains every particle.  One
+// wavefront per leaf.
+__global__ __launch_bounds__(256) void leafBoxKernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, const uint32_t* __restrict__ layout,
+                                                     int nLeaves, double* __restrict__ lb)
+{
+    const int l = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (l >= nLeaves) return;
+    const uint32_t p0 = layout[l], p1 = layout[l + 1];
+    const double* c[3] = {x, y, z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        double lo = INFINITY, hi = -INFINITY;
+        for (uint32_t p = p0 + laneId(); p < p1; p += kWave)
+        {
+            const double v = c[a][p];
+            lo = fmin(lo, v);
+            hi = fmax(hi, v);
+        }
+        lo = -waveReduceMax(-lo);
+        hi = waveReduceMax(hi);
+        if (laneId() == 0)
+        {
+            double cen = 0.0, half = 0.0;
+            if (p1 > p0)
+            {
+                cen = 0.5 * (lo + hi);
+                half = 0.5 * (hi - lo) + 0x1p-50 * fmax(fabs(lo), fabs(hi));
+            }
+            lb[(size_t)a * nLeaves + l] = cen;
+            lb[(size_t)(3 + a) * nLeaves + l] = half;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void layoutKernel(const uint64_t* __restrict__ keys,
                                                     const uint64_t* __restrict__ leafStart,
                                                     const int* __restrict__ leafSet, int nLeaves,
@@ -746,6 +783,9 @@ void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch
     uint32_t* layout = s.layout.get<uint32_t>(L + 2);
     layoutKernel<<<iceil(L + 1, 256), 256, 0, st>>>(keys, t.leafStart, t.leafSet, L, s.aliveFirst.as<int>(),
                                                     s.alive.as<int>(), layout, na);
+    LAUNCH_CHECK();
+    double* lbox = s.leafBox.get<double>((size_t)6 * L + 1);
+    leafBoxKernel<<<iceil(L, 4), 256, 0, st>>>(s.x.as<double>(), s.y.as<double>(), s.z.as<double>(), layout, L, lbox);
     LAUNCH_CHECK();
     int* pleaf = s.particleLeaf.get<int>(std::max(n, 1));
     if (na > 0)
