@@ -1,30 +1,17 @@
 /*
- * density.hip -- the post-convergence gradient sweep and small helpers of the solve (the
+ * density.hip -- the gradient sweep after convergence and small helpers of the solve (the
  * Newton-Raphson pass itself is in nr.hip).
  *
- * Restates src/density_base.cu:237-477 (density sums and NR update), :479-714 (gradient
- * sweep) and :856-886 (compaction) for gfx950.
+ * Restates src/density_base.cu:479-714 (gradient sweep: rho and d(rho)/d(h) at the final h,
+ * div v, the velocity-gradient matrix through the inverse of the r-tensor, the xi limiter and
+ * d(div v)/dt) for gfx950, in FP64 as the reference.
  *
- * Mapping: one wavefront per i-leaf (bucket 64 = wave64), lane = particle of the leaf.
- * The leaf's neighbour-leaf list is consumed 64 entries at a time: each lane reads one
- * j-leaf's particle range, a wave scan of the range lengths lays the j-particles out as
- * consecutive slots, and the slots are staged 64 at a time into an LDS tile with one
- * coalesced load per lane (j-leaf particles are contiguous in Hilbert order).  Every lane
- * then sweeps the tile with broadcast LDS reads.
- *
- * Precision forms:
- *   plain  (Real = double)  the reference's FP64 arithmetic;
- *   mixed  (Real = float)   positions enter the pair loop as FP32 offsets from the i-leaf's
- *                           geometric centre (formed in FP64, rounded once per j per tile),
- *                           the kernel sums are FP32 over each tile of <= 64 neighbours and
- *                           tile sums are added with Kahan compensation; the NR update of h,
- *                           the convergence test and the closure rho(h) stay FP64.
- *
- * Semantics of the active set (src/density_base.cu:856-886): a particle iterates until it
- * converges and is never revived (FLAT_PARTICLE); in WARP_PER_LEAF mode every particle of
- * an active leaf is recomputed but only the still-active ones can leave the active set
- * (src/density_base.cu:361-364).  Active leaves are the leaves holding an active particle,
- * compacted in leaf order, which equals the reference's sort + unique of their leaves.
+ * Mapping: one wavefront per leaf, its particles up to 64 at a time, g = floor(64 / c) lanes
+ * per particle (below).  The leaf's neighbour-leaf list is consumed 64 entries at a time: each
+ * lane reads one j-leaf's particle range, a wave scan of the range lengths lays the
+ * j-particles out as consecutive slots, and the slots are staged into an LDS tile with one
+ * coalesced load per lane (j-leaf particles are contiguous in Hilbert order).  Every lane then
+ * sweeps its slots of the tile with broadcast LDS reads.
  */
 #include "density.hpp"
 #include "prims.hpp"
@@ -82,8 +69,8 @@ __device__ inline double xiLimiter(const double g[9])
 // One wavefront per leaf, the leaf's particles up to 64 at a time.  With c particles in a
 // chunk, each takes g = floor(64 / c) lanes that split every tile (lane s of a group sweeps
 // slots s, s + g, ...), then a group reduction closes the particle: small leaves keep the
-// whole wavefront busy.  A list entry is staged only if its leaf box lies within the support
-// R h_i of some particle of the chunk (pointNodeGap2): the reference's list reaches R hmax
+// whole wavefront busy.  A list entry is staged only if its leaf's particles (their tight box)
+// lie within the support R h_i of some particle of the chunk: the reference's list reaches R hmax
 // around the whole leaf box, most of which no particle's own support touches, and a leaf the
 // test drops holds no pair with q < R, so the pair sums are unchanged (for g = 1 even their
 // order is: the list order).
