@@ -51,8 +51,8 @@ __global__ __launch_bounds__(256) void forcePrepKernel(const double* __restrict_
 
 // One wavefront per i-leaf, the leaf's particles up to 64 at a time, g = floor(64 / c) lanes
 // per particle splitting every tile (lane s of a group takes slots s, s + g, ...), then a group
-// reduction; small leaves keep the wavefront busy.  A list entry is staged only if its leaf box
-// lies within R max(h_i, hmax of the j-leaf) of some particle of the chunk: a dropped leaf holds
+// reduction; small leaves keep the wavefront busy.  A list entry is staged only if its leaf's
+// tight box lies within R max(h_i, hmax of the j-leaf) of some particle of the chunk: a dropped leaf holds
 // no pair inside either support, so the pair sums are unchanged (for g = 1 even their order).
 template<bool PER, bool DISC, bool FULL>
 __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)

@@ -443,7 +443,7 @@ void densityKernel(DensArgs A)
                     }
                 }
                 waveLdsSync();
-                // with culling, a list entry is staged only if its leaf box lies within the support
+                // with culling, a list entry is staged only if its leaf's tight box lies within the support
                 // of one of the c particles (a dropped leaf holds no pair with q < R for them)
                 auto keep = [&](int jl) {
                     if constexpr (CULL)

@@ -65,31 +65,6 @@ __device__ inline void forEachTileIf(int* __restrict__ segStart, int* __restrict
     }
 }
 
-// Squared gap between the point p and the box of node `node` (centre c, half-size s), the box
-// widened by two finest cells per axis, nearest periodic image per axis.  A particle lies
-// within one finest cell of its node's box (the key quantisation floor(x m) - lo m truncates
-// up to one cell low), so every particle of the node is at least this far from p: a node
-// whose gap exceeds R h (with a relative slack for rounding) holds no pair inside that support.
-template<bool PER>
-__device__ inline double pointNodeGap2(double px, double py, double pz, const double* __restrict__ cx,
-                                       const double* __restrict__ cy, const double* __restrict__ cz,
-                                       const double* __restrict__ sx, const double* __restrict__ sy,
-                                       const double* __restrict__ sz, int node, const DBox& box)
-{
-    const double p[3] = {px, py, pz};
-    const double c[3] = {cx[node], cy[node], cz[node]}, sh[3] = {sx[node], sy[node], sz[node]};
-    double d2 = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-    {
-        double d = c[a] - p[a];
-        if constexpr (PER) d -= box.l[a] * rint(d * box.il[a]);
-        const double v = fmax(fabs(d) - sh[a] - 2.0 * kFinestCell * box.l[a], 0.0);
-        d2 += v * v;
-    }
-    return d2;
-}
-
 constexpr double kCullSlack = 1.0 + 1e-5;
 
 // Squared gap between the point p and the tight box of leaf jl's particles (lb: centres x, y, z
