@@ -84,13 +84,30 @@ def periodic_lattice(npartx=12):
     return c, box
 
 
-def test_periodic_box():
+# every list through the block walk and every heavy item split (the periodic instances of both)
+FORCED_PATHS = {"list_heavy_steps": 1, "list_expand_leaves": 0, "split_len": 8}
+DEFAULT_PATHS = {"list_heavy_steps": 0, "list_expand_leaves": -1, "split_len": -1}
+
+
+@pytest.fixture
+def _default_paths():
+    yield
+    for kernel in ("cubic", "quintic"):
+        lib = Library(kernel)
+        for k, val in DEFAULT_PATHS.items():
+            lib.set_option(k, val)
+
+
+@pytest.mark.parametrize("paths", ["default", "forced"])
+def test_periodic_box(paths, _default_paths):
     c, box = periodic_lattice()
     n = c["n"]
     rng = np.random.default_rng(6)
     v = rng.normal(size=(3, n)) * 0.1
     lib = Library("cubic")
     lib.set_option("precision", 0)
+    for k, val in (FORCED_PATHS if paths == "forced" else DEFAULT_PATHS).items():
+        lib.set_option(k, val)
     out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], hfact=1.2, v=v, periodic_box=box)
     o = O.Oracle(quintic=False)
     o.set_particles(c["x"], c["y"], c["z"], c["h"], v=v, a=np.zeros((3, n)))

@@ -27,8 +27,22 @@ def ensemble():
     return sets
 
 
-def run_batch(sets, precision):
+FORCED_PATHS = {"list_heavy_steps": 1, "list_expand_leaves": 0, "split_len": 8}
+DEFAULT_PATHS = {"list_heavy_steps": 0, "list_expand_leaves": -1, "split_len": -1}
+
+
+@pytest.fixture(autouse=True)
+def _default_paths():
+    yield
     lib = Library("cubic")
+    for k, val in DEFAULT_PATHS.items():
+        lib.set_option(k, val)
+
+
+def run_batch(sets, precision, paths=None):
+    lib = Library("cubic")
+    for k, val in (paths or DEFAULT_PATHS).items():
+        lib.set_option(k, val)
     lib.set_option("precision", precision)
     lib.set_option("max_iter", 100)
     lib.set_option("tree_only", 0)
@@ -42,10 +56,12 @@ def run_batch(sets, precision):
     return lib, iters, n
 
 
+@pytest.mark.parametrize("paths", ["default", "forced"])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_batch_matches_single_set_solves(precision):
+def test_batch_matches_single_set_solves(precision, paths):
+    """forced: every list through the block walk and every heavy item split over wavefronts."""
     sets = ensemble()
-    lib, iters, n = run_batch(sets, precision)
+    lib, iters, n = run_batch(sets, precision, FORCED_PATHS if paths == "forced" else None)
     h = lib.download(HSML, n)[0]
     d = lib.download(DENS_OUT, n)
     leaf_set = lib.array("leafSet")
