@@ -331,6 +331,8 @@ void densityKernel(DensArgs A)
     // mixed whole-leaf items in work units: the batch's particles (fixed-point offsets, hsr bits; hsrLo)
     __shared__ int4   ptab[kMixed ? kNrWaves : 1][kMixed ? kWave : 1];
     __shared__ float  ptabLo[kMixed ? kNrWaves : 1][kMixed ? kWave : 1];
+    // plain whole-leaf items in work units (culling instances): positions from cpw, 1/h^2 here
+    __shared__ double ptabH[(!kMixed && CULL) ? kNrWaves : 1][(!kMixed && CULL) ? kWave : 1];
     __shared__ double tx[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
     __shared__ double ty[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
     __shared__ double tz[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
@@ -479,8 +481,8 @@ void densityKernel(DensArgs A)
                     if constexpr (kMixed) tq[w][l] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);
                     else { tx[w][l] = 1e200; ty[w][l] = 1e200; tz[w][l] = 1e200; }
                 };
-                const int U = (kMixed && A.units) ? (int)kUnitShapes.U[c] : 1;
-                if constexpr (kMixed)
+                const int U = ((kMixed || CULL) && A.units) ? (int)kUnitShapes.U[c] : 1;
+                if constexpr (kMixed || CULL)
                 {
                     if (U > 1)
                     {
@@ -496,11 +498,18 @@ void densityKernel(DensArgs A)
                         const double hcur = own ? A.h[i] : 1.0;
                         if (own)
                         {
-                            const double hsd = (1.0 / (hcur * hcur)) * invS2;
-                            const float hsr = float(hsd);
-                            ptab[w][lane] = make_int4(toFixed(A.x[i] - c0), toFixed(A.y[i] - c1), toFixed(A.z[i] - c2),
-                                                      __float_as_int(hsr));
-                            ptabLo[w][lane] = float(hsd - (double)hsr);
+                            if constexpr (kMixed)
+                            {
+                                const double hsd = (1.0 / (hcur * hcur)) * invS2;
+                                const float hsr = float(hsd);
+                                ptab[w][lane] = make_int4(toFixed(A.x[i] - c0), toFixed(A.y[i] - c1),
+                                                          toFixed(A.z[i] - c2), __float_as_int(hsr));
+                                ptabLo[w][lane] = float(hsd - (double)hsr);
+                            }
+                            else
+                            {
+                                ptabH[w][lane] = 1.0 / (hcur * hcur);   // positions: cpw[w][0..2][lane]
+                            }
                         }
                         waveLdsSync();
                         const int u0 = lane * U;
@@ -517,15 +526,32 @@ void densityKernel(DensArgs A)
                                 {
                                     const int pu = (int)(((uint32_t)u * mS) >> 16);
                                     const int sg = u - pu * S;
-                                    const int4 pp = ptab[w][pu];
-                                    double sv = 0.0;
-                                    float sp = 0.0f;
+                                    double sv = 0.0, sq = 0.0;
                                     int acc = 0;
-                                    sweepFixed(&tq[w][sg], S, steps, pp.x, pp.y, pp.z, __int_as_float(pp.w),
-                                               ptabLo[w][pu], sv, sp, acc);
+                                    if constexpr (kMixed)
+                                    {
+                                        const int4 pp = ptab[w][pu];
+                                        float sp = 0.0f;
+                                        sweepFixed(&tq[w][sg], S, steps, pp.x, pp.y, pp.z, __int_as_float(pp.w),
+                                                   ptabLo[w][pu], sv, sp, acc);
+                                        sq = (double)sp;
+                                    }
+                                    else
+                                    {
+                                        // slots sg + t S, t < steps: at most slot S - 1 + (ceil(64 / S) - 1) S < 96
+                                        const double xu = cpw[w][0][pu], yu = cpw[w][1][pu], zu = cpw[w][2][pu];
+                                        const double hsu = ptabH[w][pu];
+#pragma unroll 4
+                                        for (int t = 0; t < steps; ++t)
+                                        {
+                                            const int kk = sg + t * S;
+                                            pairPlain<PER>(xu - tx[w][kk], yu - ty[w][kk], zu - tz[w][kk], box, hsu, sv, sq,
+                                                           acc);
+                                        }
+                                    }
                                     const uint32_t cn = sg == 0 ? (uint32_t)nt : 0u;
-                                    if (pu == pA) { a0 += sv; a1 += (double)sp; accA += acc; candA += cn; }
-                                    else          { b0 += sv; b1 += (double)sp; accB += acc; candB += cn; }
+                                    if (pu == pA) { a0 += sv; a1 += sq; accA += acc; candA += cn; }
+                                    else          { b0 += sv; b1 += sq; accB += acc; candB += cn; }
                                 }
                             }
                         };
@@ -544,35 +570,61 @@ void densityKernel(DensArgs A)
                         }
                         // each lane's partial sums through the tile area (free until the next
                         // batch); particle r's units lie on lanes floor(r S / U) .. floor((r S + S - 1) / U)
-                        // slots 2 l, 2 l + 1 of the 128: lane l's (a0, a1), (b0, b1) as bit patterns
-                        auto pack = [](double x, double y) {
-                            const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
-                            return make_int4((int)bx, (int)(bx >> 32), (int)by, (int)(by >> 32));
-                        };
-                        auto lo64 = [](const int4& v) {
-                            return __longlong_as_double(((long long)v.y << 32) | (long long)(uint32_t)v.x);
-                        };
-                        auto hi64 = [](const int4& v) {
-                            return __longlong_as_double(((long long)v.w << 32) | (long long)(uint32_t)v.z);
-                        };
-                        waveLdsSync();
-                        tq[w][2 * lane] = pack(a0, a1);
-                        tq[w][2 * lane + 1] = pack(b0, b1);
                         waveLdsSync();
                         double s0 = 0, s1 = 0;
-                        if (own)
+                        if constexpr (kMixed)
                         {
-                            const int l0 = (lane * S) / U, l1 = (lane * S + S - 1) / U;
-                            for (int l = l0; l <= l1; ++l)
+                            // slots 2 l, 2 l + 1 of the 128: lane l's (a0, a1), (b0, b1) as bit patterns
+                            auto pack = [](double x, double y) {
+                                const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+                                return make_int4((int)bx, (int)(bx >> 32), (int)by, (int)(by >> 32));
+                            };
+                            auto lo64 = [](const int4& v) {
+                                return __longlong_as_double(((long long)v.y << 32) | (long long)(uint32_t)v.x);
+                            };
+                            auto hi64 = [](const int4& v) {
+                                return __longlong_as_double(((long long)v.w << 32) | (long long)(uint32_t)v.z);
+                            };
+                            tq[w][2 * lane] = pack(a0, a1);
+                            tq[w][2 * lane + 1] = pack(b0, b1);
+                            waveLdsSync();
+                            if (own)
                             {
-                                const int pl = (int)(((uint32_t)(l * U) * mS) >> 16);
-                                const int4 v = tq[w][2 * l + (pl == lane ? 0 : 1)];
-                                s0 += lo64(v);
-                                s1 += hi64(v);
+                                const int l0 = (lane * S) / U, l1 = (lane * S + S - 1) / U;
+                                for (int l = l0; l <= l1; ++l)
+                                {
+                                    const int pl = (int)(((uint32_t)(l * U) * mS) >> 16);
+                                    const int4 v = tq[w][2 * l + (pl == lane ? 0 : 1)];
+                                    s0 += lo64(v);
+                                    s1 += hi64(v);
+                                }
                             }
+                            waveLdsSync();
+                            tq[w][kWave + lane] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);   // the padding back
                         }
-                        waveLdsSync();
-                        tq[w][kWave + lane] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);   // the padding back
+                        else
+                        {
+                            // tx slots 2 l, 2 l + 1: lane l's (a0, a1); ty: its (b0, b1)
+                            tx[w][2 * lane] = a0;
+                            tx[w][2 * lane + 1] = a1;
+                            ty[w][2 * lane] = b0;
+                            ty[w][2 * lane + 1] = b1;
+                            waveLdsSync();
+                            if (own)
+                            {
+                                const int l0 = (lane * S) / U, l1 = (lane * S + S - 1) / U;
+                                for (int l = l0; l <= l1; ++l)
+                                {
+                                    const int pl = (int)(((uint32_t)(l * U) * mS) >> 16);
+                                    const double* src = pl == lane ? &tx[w][2 * l] : &ty[w][2 * l];
+                                    s0 += src[0];
+                                    s1 += src[1];
+                                }
+                            }
+                            waveLdsSync();
+                            tx[w][kWave + lane] = 1e200;   // the padding back
+                            ty[w][kWave + lane] = 1e200;
+                        }
                         waveLdsSync();
                         if (nSub > 1u && !splitCombine(A, subInfo, own ? lane : -1, s0, s1)) continue;
                         double wsum, gsum;
