@@ -220,30 +220,31 @@ __device__ inline void sweepFixed(const int4* ptr, int stride, int steps, int xq
 // c = 33.  Instead each particle's tile is cut into S units (slots s, s + S, ...: ceil(64 / S)
 // steps each) and every lane takes U consecutive units of the c S; a tile then costs U unit
 // sweeps plus a per-unit overhead instead of ceil(64 / g) steps.  S >= U, so a lane's units
-// belong to at most two particles.  The shape is chosen per c by the estimated cost (41 VALU
-// per step, 30 per unit, 8 per tile of the plain mapping), U in {2, 3}, S <= 32 (so that a
+// belong to at most two particles.  The shape is chosen per c by an estimated cost (41 per
+// step, `unit` per unit, 8 per tile of the plain mapping), U in {2, 3, 4}, S <= 32 (so that a
 // sweep reads no further than slot 2 S + 62 of the 128-slot tile); U = 1 keeps the plain
-// mapping.
+// mapping.  The per-unit cost that measured fastest on the 8M Plummer mixture is 15 for the
+// mixed sweep and 5 for the FP64 one, whose steps cost more against the same overhead.
 struct UnitShapes
 {
     unsigned char U[kWave + 1];
     unsigned char S[kWave + 1];
 };
 
-constexpr UnitShapes makeUnitShapes()
+constexpr UnitShapes makeUnitShapes(int unit)
 {
     UnitShapes t{};
-    constexpr int kStep = 41, kUnit = 30, kTile = 8;
+    constexpr int kStep = 41, kTile = 8;
     for (int c = 1; c <= kWave; ++c)
     {
         const int g = kWave / c;
         const int plain = ((kWave + g - 1) / g) * kStep + kTile;
         int bestU = 1, bestS = g, best = plain;
-        for (int U = 2; U <= 3; ++U)
+        for (int U = 2; U <= 4; ++U)
             for (int S = U; S <= 32; ++S)
             {
                 if (c * S > kWave * U || c * S <= kWave * (U - 1)) continue;
-                const int cost = U * (((kWave + S - 1) / S) * kStep + kUnit);
+                const int cost = U * (((kWave + S - 1) / S) * kStep + unit);
                 if (cost < best && 100 * cost < 97 * plain)
                 {
                     best = cost;
@@ -257,7 +258,8 @@ constexpr UnitShapes makeUnitShapes()
     return t;
 }
 
-__constant__ constexpr UnitShapes kUnitShapes = makeUnitShapes();
+__constant__ constexpr UnitShapes kUnitShapesMixed = makeUnitShapes(15);
+__constant__ constexpr UnitShapes kUnitShapesPlain = makeUnitShapes(5);
 
 // The end of one sub-item of a split work item: particle r's sums (lanes with r >= 0) go to the
 // item's slot; the wavefront whose sub-item completes the item (the count of finished sub-items
@@ -481,7 +483,8 @@ void densityKernel(DensArgs A)
                     if constexpr (kMixed) tq[w][l] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);
                     else { tx[w][l] = 1e200; ty[w][l] = 1e200; tz[w][l] = 1e200; }
                 };
-                const int U = ((kMixed || CULL) && A.units) ? (int)kUnitShapes.U[c] : 1;
+                const UnitShapes& shapes = kMixed ? kUnitShapesMixed : kUnitShapesPlain;
+                const int U = ((kMixed || CULL) && A.units) ? (int)shapes.U[c] : 1;
                 if constexpr (kMixed || CULL)
                 {
                     if (U > 1)
@@ -489,7 +492,7 @@ void densityKernel(DensArgs A)
                         // ---- work units: each particle's tile in S units, U consecutive units
                         // per lane, a lane's units summed per particle (A: its first, B: the
                         // next), the particle's sums gathered from its lanes at the end ----
-                        const int S = kUnitShapes.S[c];
+                        const int S = shapes.S[c];
                         const int nU = c * S;
                         const uint32_t mS = (65536u + (uint32_t)S - 1u) / (uint32_t)S;   // u / S = (u mS) >> 16, u < 1024
                         const bool own = lane < c;   // lane r holds particle r of the batch
