@@ -223,7 +223,8 @@ struct Reach
 
 // out(root) of an accepted internal node by the 8 lanes of a group (lane o tests child o of
 // the current node): the count, and without COUNT the entries at jList[base ...] below limit.
-// Stops after stepCap steps with capped set.
+// Stops after stepCap steps with capped set.  The stack holds each pushed node's first child
+// (read with the accept test), so a step waits for one round of loads, not two.
 template<bool COUNT, class R>
 __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int root, int* __restrict__ stk,
                                      uint32_t base, uint32_t limit, int stepCap, bool& capped)
@@ -232,7 +233,7 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
     const int shift = (laneId() >> 3) << 3;
     const uint32_t below = (1u << o) - 1u;
     uint32_t count = 0;
-    int sp = 0, node = root, steps = 0;
+    int sp = 0, first = A.nodeChild[root], steps = 0;
     capped = false;
     while (true)
     {
@@ -241,7 +242,7 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
             capped = true;
             break;
         }
-        const int  c    = A.nodeChild[node] + o;
+        const int  c    = first + o;
         const bool acc  = inRange(A, c);
         const int  cref = A.nodeChild[c];
         const bool isLeaf = acc && cref < 0;
@@ -258,7 +259,7 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
         const int push = __popc(mI);
         if (sp + push <= kStack)
         {
-            if (isInt) stk[sp + __popc(mI & below)] = c;
+            if (isInt) stk[sp + __popc(mI & below)] = cref;
             sp += push;
         }
         else if (o == 0)
@@ -267,7 +268,7 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
         }
         waveLdsSync();
         if (sp == 0) break;
-        node = stk[--sp];
+        first = stk[--sp];
         waveLdsSync();
     }
     return count;
