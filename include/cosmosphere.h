@@ -94,11 +94,11 @@ int cosmo_ext_solve_dens_h(int n, double pmass, int mode, int withGradients, int
  * "loop" (NR passes: 0 host-driven, 1 device-driven, the default: counts in device memory,
  * "graph_passes" passes per HIP graph replay when "graph" is 1, persistent density grid of
  * "loop_blocks_scale" / 4 times the resident blocks, cost-ordered work queue when "lpt" is 1),
- * "bucket" (leaf bucket size, default 64), "units" (1, the default: mixed NR whole-leaf items cut
- * each particle's tile into work units spread evenly over the 64 lanes where one particle per
- * lane group would leave lanes idle; 0: one group of floor(64 / c) lanes per particle),
- * "cull" (1, the default: NR whole-leaf items stage only the
- * list entries some particle's own support reaches; 0: every entry of the tight list),
+ * "bucket" (leaf bucket size, default 64), "units" (1, the default: NR whole-leaf items (mixed, and
+ * FP64 with culling on) cut each particle's tile into work units spread evenly over the 64 lanes
+ * where one particle per lane group would leave lanes idle; 0: one group of floor(64 / c) lanes
+ * per particle), "cull" (1, the default: NR whole-leaf items stage only the list entries some
+ * particle's own support reaches; 0: every entry of the tight list),
  * "mapping" (NR work items: 0 auto = one wavefront per leaf, one per particle once at most
  * "sparse" particles of the leaf remain; 1 = always one per leaf; 2 = always one per
  * particle), "sparse" (default 16), "closure" (-1 auto = on for mixed, 0 off, 1 on: rho and
