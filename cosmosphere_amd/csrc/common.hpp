@@ -112,10 +112,12 @@ constexpr double kWw = 0.25, kGp = 0.75, kGw = 0.75;
 template<class T>
 __host__ __device__ inline void kernVD(T q, T q2, T& V, T& qD)
 {
-    const bool inner = q < T(1);
+    const bool inner = q2 < T(1);   // q < 1 (at q = 1 both forms hold), known before the root
 #ifdef COSMO_KERNEL_QUINTIC
+    // b is used for q >= 1 only, where 2 - q <= 1: clamped to [0, 1] (one VALU op with the
+    // clamp modifier) it equals the reference's (2 - q)_+
     const T a = T(3) - q;
-    const T b = fmax(T(2) - q, T(0));
+    const T b = fmin(fmax(T(2) - q, T(0)), T(1));
     const T a2 = a * a, b2 = b * b;
     const T a4 = a2 * a2, b4 = b2 * b2;
     const T vOut = fma(T(-6), b4 * b, a4 * a);

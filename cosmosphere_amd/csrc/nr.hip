@@ -166,15 +166,16 @@ __device__ inline void pairPlain(double dx, double dy, double dz, const DBox& bo
 
 // Mixed pair term: FP32 separation (exact to one rounding), r^2, support test, q and kernel
 // brackets V, q D (kernVD), when q < R, added to the caller's sums sv (V) and sp (q D).
-// q^2 = r^2 (hsr + hsrLo), hsr + hsrLo = 1/h^2 in units of the separation's.
+// q^2 = r^2 hsr, hsr = 1/h^2 (in units of the separation's) rounded to FP32: its relative
+// rounding, the same for all of a particle's pairs, is taken out of the W sum once per
+// particle (particleSums).
 template<class VSum>
-__device__ inline void pairMixed(float dx, float dy, float dz, float hsr, float hsrLo, VSum& sv, float& sp, int& acc)
+__device__ inline void pairMixed(float dx, float dy, float dz, float hsr, VSum& sv, float& sp, int& acc)
 {
     const float r2 = dx * dx + dy * dy + dz * dz;
-    float q2 = r2 * hsr;
+    const float q2 = r2 * hsr;
     if (q2 < float(kRad2))
     {
-        q2 = fmaf(r2, hsrLo, q2);
         const float q = __builtin_amdgcn_sqrtf(q2);
         float V, qD;
         kernVD<float>(q, q2, V, qD);
@@ -196,7 +197,7 @@ constexpr int kVChunk = 16;
 // of them), each slot read one step ahead: W brackets into sv (FP32 partials of <= kVChunk
 // pairs), q D brackets into sp (the caller's FP32 partial).
 __device__ inline void sweepFixed(const int4* ptr, int stride, int steps, int xq, int yq, int zq, float hsr,
-                                  float hsrLo, double& sv, float& sp, int& acc)
+                                  double& sv, float& sp, int& acc)
 {
     int4 nx = *ptr;
     for (int t0 = 0; t0 < steps; t0 += kVChunk)
@@ -209,10 +210,19 @@ __device__ inline void sweepFixed(const int4* ptr, int stride, int steps, int xq
             const int4 q = nx;
             ptr += stride;
             nx = *ptr;
-            pairMixed((float)(xq - q.x), (float)(yq - q.y), (float)(zq - q.z), hsr, hsrLo, v, sp, acc);
+            pairMixed((float)(xq - q.x), (float)(yq - q.y), (float)(zq - q.z), hsr, v, sp, acc);
         }
         sv += (double)v;
     }
+}
+
+// The mixed form's 1/h^2 as the FP32 hsr of the pair loop and the relative rounding eps of
+// hsr (1 + eps) = hsd (|eps| <= 2^-24).
+__device__ inline float mixedHsr(double hsd, double& eps)
+{
+    const float hsr = float(hsd);
+    eps = (hsd - (double)hsr) / (double)hsr;
+    return hsr;
 }
 
 // Work units of a whole-leaf batch of c particles.  With g = floor(64 / c) lanes per particle,
@@ -295,18 +305,23 @@ __device__ inline bool splitCombine(const DensArgs& A, uint32_t subInfo, int r, 
     return true;
 }
 
-// Particle sums from the accumulators: (sum W, sum -q dW/dq - 3W).
+// Particle sums from the accumulators: (sum W, sum -q dW/dq - 3W).  Mixed: the pair loop took
+// q at 1/h^2 = hsr (1 + eps), so each q is (1 + eps/2) short; to first order the W brackets at
+// the exact q sum to s0 + (eps/2) sum q V'(q) = s0 - (eps/2) (kGp / kWw) s1 (q V' = -(kGp / kWw)
+// q D for both kernels); the remainder is O(eps^2) = 4e-15.
 template<bool MIXED>
-__device__ inline void particleSums(double s0, double s1, double& wsum, double& gsum)
+__device__ inline void particleSums(double s0, double s1, double eps, double& wsum, double& gsum)
 {
 #pragma clang fp contract(off)
     if constexpr (MIXED)
     {
-        wsum = kWw * s0;
-        gsum = kGp * s1 - kGw * s0;
+        const double v = s0 - (0.5 * eps) * (kGp / kWw) * s1;
+        wsum = kWw * v;
+        gsum = kGp * s1 - kGw * v;
     }
     else
     {
+        (void)eps;
         wsum = s0;
         gsum = s1;
     }
@@ -330,9 +345,8 @@ void densityKernel(DensArgs A)
     // whole-leaf tiles: mixed = fixed-point offsets (x, y, z, pad), one 16-byte LDS read per j;
     // plain = FP64 positions
     __shared__ int4   tq[kMixed ? kNrWaves : 1][kMixed ? 2 * kWave : 1];
-    // mixed whole-leaf items in work units: the batch's particles (fixed-point offsets, hsr bits; hsrLo)
+    // mixed whole-leaf items in work units: the batch's particles (fixed-point offsets, hsr bits)
     __shared__ int4   ptab[kMixed ? kNrWaves : 1][kMixed ? kWave : 1];
-    __shared__ float  ptabLo[kMixed ? kNrWaves : 1][kMixed ? kWave : 1];
     // plain whole-leaf items in work units (culling instances): positions from cpw, 1/h^2 here
     __shared__ double ptabH[(!kMixed && CULL) ? kNrWaves : 1][(!kMixed && CULL) ? kWave : 1];
     __shared__ double tx[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
@@ -365,8 +379,8 @@ void densityKernel(DensArgs A)
             const uint8_t st = MODE == 2 ? 0 : A.state[i];
             const double xi = A.x[i], yi = A.y[i], zi = A.z[i], hcur = A.h[i];
             const double hsi = 1.0 / (hcur * hcur);
-            const float hsr = float(hsi);
-            const float hsrLo = float(hsi - (double)hsr);
+            double eps = 0.0;
+            const float hsr = mixedHsr(hsi, eps);
             const double reach2 = (kRad * hcur) * (kRad * hcur) * kPruneSlack;
             double s0 = 0, s1 = 0;
             int acc = 0;
@@ -378,7 +392,7 @@ void densityKernel(DensArgs A)
                 {
                     imageFP64<PER>(dx, dy, dz, box);
                     float sp = 0.0f;   // one pair per lane per tile
-                    pairMixed(float(dx), float(dy), float(dz), hsr, hsrLo, s0, sp, acc);
+                    pairMixed(float(dx), float(dy), float(dz), hsr, s0, sp, acc);
                     s1 += (double)sp;
                 }
                 else
@@ -395,7 +409,7 @@ void densityKernel(DensArgs A)
             cc[slot + 1] += cand;
             if (nSub > 1u && !splitCombine(A, subInfo, lane == 0 ? 0 : -1, s0, s1)) return;
             double wsum, gsum;
-            particleSums<kMixed>(s0, s1, wsum, gsum);
+            particleSums<kMixed>(s0, s1, eps, wsum, gsum);
             if (lane == 0) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
         }
         else
@@ -424,8 +438,6 @@ void densityKernel(DensArgs A)
                 v = fmin(fmax(v, -1073741824.0), 1073741824.0);   // +-2^30
                 return (int)v;
             };
-            if constexpr (kMixed) tq[w][kWave + lane] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);
-            else { tx[w][kWave + lane] = 1e200; ty[w][kWave + lane] = 1e200; tz[w][kWave + lane] = 1e200; }
             for (uint32_t q0 = p0; q0 < p1; q0 += kWave)
             {
                 const uint32_t ic = q0 + lane;
@@ -464,27 +476,43 @@ void densityKernel(DensArgs A)
                         return true;
                     }
                 };
-                auto load = [&](int l, int j) {
+                const UnitShapes& shapes = kMixed ? kUnitShapesMixed : kUnitShapesPlain;
+                const int U = ((kMixed || CULL) && A.units) ? (int)shapes.U[c] : 1;
+                // FP64 tiles in sweep order: slot l at position (l mod Sl) Tl + floor(l / Sl), so
+                // that the slots l0, l0 + Sl, l0 + 2 Sl, ... of one work unit (Sl = S) or of one
+                // lane of a group (Sl = g) sit at consecutive positions (LDS reads with immediate
+                // offsets: 1.6% off the FP64 density kernel of the 8M Plummer mixture).  Mixed
+                // tiles keep slot order (position = slot), which measured 3% faster than sweep
+                // order there.
+                const int Sl = U > 1 ? (int)shapes.S[c] : g;
+                const int Tl = (kWave + Sl - 1) / Sl;
+                const int pos = kMixed ? lane : (lane % Sl) * Tl + lane / Sl;   // of slot `lane`, staged by this lane
+                auto load = [&](int, int j) {
                     double xj = A.x[j], yj = A.y[j], zj = A.z[j];
                     if constexpr (kMixed)
                     {
                         xj -= c0; yj -= c1; zj -= c2;
                         imageFP64<PER>(xj, yj, zj, box);
-                        tq[w][l] = make_int4(toFixed(xj), toFixed(yj), toFixed(zj), 0);
+                        tq[w][pos] = make_int4(toFixed(xj), toFixed(yj), toFixed(zj), 0);
                     }
                     else
                     {
-                        tx[w][l] = xj; ty[w][l] = yj; tz[w][l] = zj;
+                        tx[w][pos] = xj; ty[w][pos] = yj; tz[w][pos] = zj;
                     }
                 };
                 // slots past the tile's end hold a j beyond every support: 2^30 fixed-point
                 // units (>= 2 range) from the centre, or 1e200 in FP64 (r^2 overflows to inf)
-                auto pad = [&](int l) {
-                    if constexpr (kMixed) tq[w][l] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);
-                    else { tx[w][l] = 1e200; ty[w][l] = 1e200; tz[w][l] = 1e200; }
+                auto padAt = [&](int k) {
+                    if constexpr (kMixed) tq[w][k] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);
+                    else { tx[w][k] = 1e200; ty[w][k] = 1e200; tz[w][k] = 1e200; }
                 };
-                const UnitShapes& shapes = kMixed ? kUnitShapesMixed : kUnitShapesPlain;
-                const int U = ((kMixed || CULL) && A.units) ? (int)shapes.U[c] : 1;
+                auto pad = [&](int) { padAt(pos); };
+                // the tiles stage slots 0 .. 63; the slots a sweep reads beyond them hold padding
+                // through the batch: mixed, slots below 2 S + 63 <= 127 (the strided sweeps with
+                // their read-ahead); FP64, slots below Sl Tl <= 95 (sweep order, the read-ahead
+                // reaches position Sl Tl <= 127 at most)
+                if constexpr (kMixed) padAt(kWave + lane);
+                else if (lane < Sl * Tl - kWave) padAt(((kWave + lane) % Sl) * Tl + (kWave + lane) / Sl);
                 if constexpr (kMixed || CULL)
                 {
                     if (U > 1)
@@ -499,15 +527,14 @@ void densityKernel(DensArgs A)
                         const uint32_t i = own ? q0 + (uint32_t)rankLane[w][lane] : p0;
                         const uint8_t st = (own && MODE != 2) ? A.state[i] : 0;
                         const double hcur = own ? A.h[i] : 1.0;
+                        double eps = 0.0;
                         if (own)
                         {
                             if constexpr (kMixed)
                             {
-                                const double hsd = (1.0 / (hcur * hcur)) * invS2;
-                                const float hsr = float(hsd);
+                                const float hsr = mixedHsr((1.0 / (hcur * hcur)) * invS2, eps);
                                 ptab[w][lane] = make_int4(toFixed(A.x[i] - c0), toFixed(A.y[i] - c1),
                                                           toFixed(A.z[i] - c2), __float_as_int(hsr));
-                                ptabLo[w][lane] = float(hsd - (double)hsr);
                             }
                             else
                             {
@@ -535,19 +562,19 @@ void densityKernel(DensArgs A)
                                     {
                                         const int4 pp = ptab[w][pu];
                                         float sp = 0.0f;
-                                        sweepFixed(&tq[w][sg], S, steps, pp.x, pp.y, pp.z, __int_as_float(pp.w),
-                                                   ptabLo[w][pu], sv, sp, acc);
+                                        sweepFixed(&tq[w][sg], S, steps, pp.x, pp.y, pp.z, __int_as_float(pp.w), sv,
+                                                   sp, acc);
                                         sq = (double)sp;
                                     }
                                     else
                                     {
-                                        // slots sg + t S, t < steps: at most slot S - 1 + (ceil(64 / S) - 1) S < 96
+                                        // slots sg + t S, t < steps <= Tl: positions sg Tl + t, below S Tl <= 95
                                         const double xu = cpw[w][0][pu], yu = cpw[w][1][pu], zu = cpw[w][2][pu];
                                         const double hsu = ptabH[w][pu];
 #pragma unroll 4
                                         for (int t = 0; t < steps; ++t)
                                         {
-                                            const int kk = sg + t * S;
+                                            const int kk = sg * Tl + t;
                                             pairPlain<PER>(xu - tx[w][kk], yu - ty[w][kk], zu - tz[w][kk], box, hsu, sv, sq,
                                                            acc);
                                         }
@@ -602,8 +629,6 @@ void densityKernel(DensArgs A)
                                     s1 += hi64(v);
                                 }
                             }
-                            waveLdsSync();
-                            tq[w][kWave + lane] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);   // the padding back
                         }
                         else
                         {
@@ -624,14 +649,11 @@ void densityKernel(DensArgs A)
                                     s1 += src[1];
                                 }
                             }
-                            waveLdsSync();
-                            tx[w][kWave + lane] = 1e200;   // the padding back
-                            ty[w][kWave + lane] = 1e200;
                         }
-                        waveLdsSync();
+                        waveLdsSync();   // the tile area is the next batch's
                         if (nSub > 1u && !splitCombine(A, subInfo, own ? lane : -1, s0, s1)) continue;
                         double wsum, gsum;
-                        particleSums<kMixed>(s0, s1, wsum, gsum);
+                        particleSums<kMixed>(s0, s1, eps, wsum, gsum);
                         if (own) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
                         continue;
                     }
@@ -646,21 +668,20 @@ void densityKernel(DensArgs A)
                     const double hsi = 1.0 / (hcur * hcur);
                     int xq = 0, yq = 0, zq = 0;
                     if constexpr (kMixed) { xq = toFixed(xi - c0); yq = toFixed(yi - c1); zq = toFixed(zi - c2); }
-                    const double hsd = hsi * invS2;
-                    const float hsr = float(hsd);
-                    const float hsrLo = float(hsd - (double)hsr);
+                    double eps = 0.0;
+                    const float hsr = mixedHsr(hsi * invS2, eps);
                     double s0 = 0, s1 = 0;
                     int acc = 0;
                     uint32_t cand = 0;
-                    // pairs with tile slots first, first + stride, ...; each slot is read one step
-                    // ahead.  Slots from nt on hold padding: [nt, 64) written with the tile, the
-                    // upper half [64, 128) once per wavefront; a group of g lanes reads at most
-                    // slot g - 1 + ceil(nt / g) g < 128.
-                    auto sweep = [&](int first, int stride, int steps) {
+                    // lane sub of a group: pairs with slots sub, sub + g, ... (FP64: positions
+                    // sub Tl, sub Tl + 1, ...), each read one step ahead.  Slots from nt on hold
+                    // padding ([nt, 64) staged with the tile, the rest for the batch); the mixed
+                    // read-ahead reaches slot g - 1 + ceil(nt / g) g < 128.
+                    auto sweep = [&](int first, int steps) {
                         if constexpr (kMixed)
                         {
                             float sp = 0.0f;
-                            sweepFixed(&tq[w][first], stride, steps, xq, yq, zq, hsr, hsrLo, s0, sp, acc);
+                            sweepFixed(&tq[w][first], g, steps, xq, yq, zq, hsr, s0, sp, acc);
                             s1 += (double)sp;
                         }
                         else
@@ -668,7 +689,7 @@ void densityKernel(DensArgs A)
     #pragma unroll 4
                             for (int t = 0; t < steps; ++t)
                             {
-                                const int k = first + t * stride;
+                                const int k = first * Tl + t;
                                 pairPlain<PER>(xi - tx[w][k], yi - ty[w][k], zi - tz[w][k], box, hsi, s0, s1, acc);
                             }
                         }
@@ -676,11 +697,11 @@ void densityKernel(DensArgs A)
                     auto consume = [&](int nt) {
                         if (g == 1)
                         {
-                            sweep(0, 1, nt);   // list order, as the reference
+                            sweep(0, nt);   // list order, as the reference
                         }
                         else
                         {
-                            sweep(sub, g, __builtin_amdgcn_readfirstlane((nt + g - 1) / g));
+                            sweep(sub, __builtin_amdgcn_readfirstlane((nt + g - 1) / g));
                         }
                         cand += sub == 0 ? (uint32_t)nt : 0u;
                     };
@@ -699,7 +720,7 @@ void densityKernel(DensArgs A)
                     }
                     if (nSub > 1u && !splitCombine(A, subInfo, (has && sub == 0) ? grp : -1, s0, s1)) continue;
                     double wsum, gsum;
-                    particleSums<kMixed>(s0, s1, wsum, gsum);
+                    particleSums<kMixed>(s0, s1, eps, wsum, gsum);
                     if (has && sub == 0) finishParticle<MODE>(A, i, iLeaf, st, wsum, gsum, hcur);
                 }
             }
