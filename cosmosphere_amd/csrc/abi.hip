@@ -344,6 +344,7 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "loop_list_cap") s.loopListCap = (long long)value;
     else if (nm == "list_heavy_steps") s.listHeavySteps = (int)value;
     else if (nm == "split_len") s.splitLen = (int)value;
+    else if (nm == "sweep_split") s.sweepSplit = std::max(0, (int)value);
     else if (nm == "list_expand_leaves") s.listExpandLeaves = (int)value;
     else
     {
@@ -407,6 +408,8 @@ extern "C" double cosmo_ext_get_stat(const char* name)
     if (nm == "sparse") return s.sparse;
     if (nm == "tail_leaves") return s.tailLeaves;
     if (nm == "loop_blocks_scale") return s.loopBlockScale;
+    if (nm == "sweep_split") return s.sweepSplit;
+    if (nm == "sweep_items") return s.sweepHeavyItems;
     return -1.0;
 }
 

@@ -7,7 +7,8 @@
  * d(div v)/dt) for gfx950, in FP64 as the reference.
  *
  * Mapping: one wavefront per leaf, its particles up to 64 at a time, g = floor(64 / c) lanes
- * per particle (below).  The leaf's neighbour-leaf list is consumed 64 entries at a time: each
+ * per particle (below); the particles of a leaf with a long list one wavefront each
+ * (SweepItems).    The leaf's neighbour-leaf list is consumed 64 entries at a time: each
  * lane reads one j-leaf's particle range, a wave scan of the range lengths lays the
  * j-particles out as consecutive slots, and the slots are staged into an LDS tile with one
  * coalesced load per lane (j-leaf particles are contiguous in Hilbert order).  Every lane then
@@ -82,10 +83,10 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
     __shared__ double tj[kDensWaves][9][kWave];
     __shared__ double cp[kDensWaves][4][kWave];   // the chunk's particles: x, y, z, (R h)^2 slack
     const int w = threadIdx.x >> 6, lane = laneId();
-    const int a = blockIdx.x * kDensWaves + w;
-    if (a >= A.nLeaves) return;
+    int a;
+    uint32_t p0, p1;
+    if (!A.items.resolve(blockIdx.x * kDensWaves + w, A.nLeaves, A.layout, a, p0, p1)) return;
     const int iLeaf = a;
-    const uint32_t p0 = A.layout[iLeaf], p1 = A.layout[iLeaf + 1];
     const uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
     const DBox box = A.boxes[A.leafSet[iLeaf]];
     const double pmass = A.pmass;
@@ -117,19 +118,26 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
                     return true;
             return false;
         };
+        // all nine loads in flight before the first LDS write (one memory round trip per tile;
+        // left to itself the scheduler pairs each two loads with their write)
         auto load = [&](int l, int j) {
-            tj[w][0][l] = A.x[j];  tj[w][1][l] = A.y[j];  tj[w][2][l] = A.z[j];
-            tj[w][3][l] = A.vx[j]; tj[w][4][l] = A.vy[j]; tj[w][5][l] = A.vz[j];
-            tj[w][6][l] = A.ax[j]; tj[w][7][l] = A.ay[j]; tj[w][8][l] = A.az[j];
+            const double f[9] = {A.x[j], A.y[j], A.z[j], A.vx[j], A.vy[j], A.vz[j], A.ax[j], A.ay[j], A.az[j]};
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) tj[w][q][l] = f[q];
         };
-        auto pair = [&](int k) {
-            double dx = xi - tj[w][0][k], dy = yi - tj[w][1][k], dz = zi - tj[w][2][k];
+        auto sep = [&](int k, double& dx, double& dy, double& dz) {
+            dx = xi - tj[w][0][k]; dy = yi - tj[w][1][k]; dz = zi - tj[w][2][k];
             if constexpr (PER)
             {
                 if (fabs(dx) > 0.5 * box.l[0]) dx = dx - box.l[0] * copysign(1.0, dx);
                 if (fabs(dy) > 0.5 * box.l[1]) dy = dy - box.l[1] * copysign(1.0, dy);
                 if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
             }
+        };
+        auto pair = [&](int k) {
+            double dx, dy, dz;
+            sep(k, dx, dy, dz);
             const double r2 = dx * dx + dy * dy + dz * dz;
             const double q2 = r2 * hsi;
             if (q2 >= kRad2) return;
@@ -154,9 +162,25 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
             ryy -= dy * ry; ryz -= dy * rz; rzz -= dz * rz;
         };
         // lane sub of a group takes slots sub, sub + g, ... below nt (g = 1: slots 0 .. nt - 1,
-        // the list order, as the reference)
+        // the list order, as the reference), in two phases: the support test of every slot into
+        // a bit mask, then the pair terms of the accepted slots in slot order.  A wavefront runs
+        // the pair terms as often as its busiest lane accepts a pair, not once per candidate
+        // (at the final h about 1 candidate in 8 is a pair); each lane's sums are unchanged.
         auto consume = [&](int nt) {
-            for (int k = sub; k < nt; k += g) pair(k);
+            uint64_t accept = 0;
+            int t = 0;
+            for (int k = sub; k < nt; k += g, ++t)
+            {
+                double dx, dy, dz;
+                sep(k, dx, dy, dz);
+                if ((dx * dx + dy * dy + dz * dz) * hsi < kRad2) accept |= 1ull << t;
+            }
+            while (accept)
+            {
+                const int tt = __builtin_ctzll(accept);
+                accept &= accept - 1;
+                pair(sub + tt * g);
+            }
         };
         forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, [](int) {}, consume);
         // group sums: a tree over lanes sub .. g - 1 of each group (lane sub = 0 ends with them)
@@ -241,11 +265,59 @@ void launchGradients(const GradArgs& a, bool periodic, hipStream_t st)
 {
     if (a.nLeaves > 0)
     {
-        const int grid = iceil(a.nLeaves, kDensWaves);
+        const int grid = iceil(a.items.count(a.nLeaves), kDensWaves);
         if (periodic) gradientKernel<true><<<grid, kDensThreads, 0, st>>>(a);
         else          gradientKernel<false><<<grid, kDensThreads, 0, st>>>(a);
         LAUNCH_CHECK();
     }
+}
+
+// Sweep items: leaf a with more than `entries` list entries and more than one particle gets
+// one item per particle.
+__global__ void sweepCountKernel(int nLeaves, const uint32_t* __restrict__ jOffset, const uint32_t* __restrict__ layout,
+                                 int entries, uint8_t* __restrict__ flag, uint32_t* __restrict__ count)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nLeaves) return;
+    const uint32_t np = layout[a + 1] - layout[a];
+    const bool heavy = entries > 0 && jOffset[a + 1] - jOffset[a] > (uint32_t)entries && np > 1u;
+    flag[a] = heavy ? 1 : 0;
+    count[a] = heavy ? np : 0u;
+}
+
+__global__ void sweepFillKernel(int nLeaves, const uint32_t* __restrict__ layout, const uint8_t* __restrict__ flag,
+                                const uint32_t* __restrict__ off, int2* __restrict__ items)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nLeaves || !flag[a]) return;
+    const uint32_t p0 = layout[a], p1 = layout[a + 1];
+    for (uint32_t p = p0; p < p1; ++p) items[off[a] + (p - p0)] = make_int2(a, (int)p);
+}
+
+SweepItems buildSweepItems(State& s, int nLeaves, int entries, hipStream_t st)
+{
+    SweepItems it;
+    s.sweepHeavyItems = 0;
+    if (nLeaves <= 0 || entries <= 0) return it;
+    uint8_t* flag = s.sweepFlag.get<uint8_t>((size_t)nLeaves);
+    uint32_t* count = s.sweepCount.get<uint32_t>((size_t)nLeaves + 1);
+    uint32_t* off = s.sweepOff.get<uint32_t>((size_t)nLeaves + 1);
+    sweepCountKernel<<<iceil(nLeaves, 256), 256, 0, st>>>(nLeaves, s.jOffset.as<uint32_t>(), s.layout.as<uint32_t>(),
+                                                          entries, flag, count);
+    LAUNCH_CHECK();
+    scanExclusive<uint32_t>(count, off, nLeaves, s.scanTmp, nullptr, st);
+    uint32_t total = 0;
+    HIP_CHECK(hipMemcpyAsync(&total, off + nLeaves, sizeof(total), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (total == 0) return it;
+    int2* items = s.sweepItems.get<int2>(total);
+    sweepFillKernel<<<iceil(nLeaves, 256), 256, 0, st>>>(nLeaves, s.layout.as<uint32_t>(), flag, off, items);
+    LAUNCH_CHECK();
+    it.nHeavy = (int)total;
+    it.heavy = items;
+    it.flag = flag;
+    s.sweepHeavyItems = (int)total;
+    return it;
 }
 
 void launchDeadGradients(int first, int n, double* divv, double* xi, double* ddivvdt, hipStream_t st)

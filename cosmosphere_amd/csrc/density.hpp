@@ -99,6 +99,38 @@ enum Counter
     kNumCounters = 7
 };
 
+// Work items of a gradient or force sweep, one wavefront each: items [0, nHeavy) are single
+// particles (heavy[k] = (leaf, particle)) of the leaves whose lists are long (flag[leaf] != 0);
+// item nHeavy + l is leaf l, skipped if flagged.  A whole-leaf wavefront runs the leaf's list
+// once per chunk of up to 64 particles; a long list (a leaf whose largest h reaches far, most of
+// it beyond the other particles' supports) then keeps one wavefront busy long after the rest of
+// the grid has drained.  Its particles spread over wavefronts finish together with the others.
+struct SweepItems
+{
+    int            nHeavy = 0;
+    const int2*    heavy = nullptr;
+    const uint8_t* flag = nullptr;
+
+    // item k -> leaf a and its particle range [p0, p1); false for a skipped item
+    __device__ bool resolve(int k, int nLeaves, const uint32_t* layout, int& a, uint32_t& p0, uint32_t& p1) const
+    {
+        if (k < nHeavy)
+        {
+            const int2 it = heavy[k];
+            a = it.x;
+            p0 = (uint32_t)it.y;
+            p1 = p0 + 1u;
+            return true;
+        }
+        a = k - nHeavy;
+        if (a >= nLeaves || (flag && flag[a])) return false;
+        p0 = layout[a];
+        p1 = layout[a + 1];
+        return true;
+    }
+    int count(int nLeaves) const { return nLeaves > 0 ? nHeavy + nLeaves : 0; }
+};
+
 struct GradArgs
 {
     int             nLeaves;
@@ -118,6 +150,7 @@ struct GradArgs
     int             lboxN;
     const double*   cx; const double* cy; const double* cz;
     const double*   sx; const double* sy; const double* sz;
+    SweepItems      items;                                    // one wavefront per item
 };
 
 struct ForceArgs
@@ -146,6 +179,7 @@ struct ForceArgs
     const double*   hmaxLeaf;                                 // reach of a j-leaf's own supports
     const float*    f32;      // mixed form: FP32 copies of hsqinv, hinv, grkfac, rhoh, rho1, pro2, cs,
     size_t          f32Stride; // alpha, pres, auterm (array k at f32 + k f32Stride)
+    SweepItems      items;    // one wavefront per item
 };
 
 // mode 0: NR on the particles in state != 0 (FLAT_PARTICLE; state 2 = closure only);
@@ -157,6 +191,9 @@ void launchDensity(const DensArgs& a, int precision, int mode, bool periodic, hi
 void launchDensityPersistent(const DensArgs& a, int precision, bool periodic, int blocks, hipStream_t st);
 int  densityPersistentBlocks(int precision, bool periodic);
 void launchGradients(const GradArgs& a, bool periodic, hipStream_t st);
+// Per-particle items of the leaves with more than `entries` list entries (0: none); reads the
+// item count back (host-synchronising).
+SweepItems buildSweepItems(State& s, int nLeaves, int entries, hipStream_t st);
 void launchDeadGradients(int first, int n, double* divv, double* xi, double* ddivvdt, hipStream_t st);
 // Per active leaf: flag (has a particle in state != 0) and its work-item count (1 whole-leaf
 // item, or one item per remaining particle when at most `sparse` remain); counters 2-4.

@@ -52,8 +52,11 @@ __global__ __launch_bounds__(256) void forcePrepKernel(const double* __restrict_
 // One wavefront per i-leaf, the leaf's particles up to 64 at a time, g = floor(64 / c) lanes
 // per particle splitting every tile (lane s of a group takes slots s, s + g, ...), then a group
 // reduction; small leaves keep the wavefront busy.  A list entry is staged only if its leaf's
-// tight box lies within R max(h_i, hmax of the j-leaf) of some particle of the chunk: a dropped leaf holds
-// no pair inside either support, so the pair sums are unchanged (for g = 1 even their order).
+// tight box lies within R max(h_i, hmax of the j-leaf) of some particle of the chunk, and of its
+// leaf a particle j only if it lies within R max(h_j, max h of the chunk) of the chunk's bounding
+// box: a dropped leaf or particle holds no pair inside either support, so the pair sums are
+// unchanged (for g = 1 even their order; the survivors fill the tiles in list order).  On the
+// 16M-particle Plummer mixture the particle test leaves a third of the candidates.
 template<bool PER, bool DISC, bool FULL>
 __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
 {
@@ -65,13 +68,14 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
     enum { X, Y, Z, H, VX, VY, VZ, HS, HI, GK, RH, R1, P2, CS, AL, U, PR, AU, NF };
     __shared__ double tf[kForceWaves][NF][kWave];
     __shared__ double cp[kForceWaves][4][kWave];   // the chunk's particles: x, y, z, R h
+    __shared__ int    jq[kForceWaves][2 * kWave];  // candidates that passed the particle test
     const int w = threadIdx.x >> 6, lane = laneId();
-    const int a = blockIdx.x * kForceWaves + w;
+    int a;
+    uint32_t p0, p1;
     unsigned long long nAcc = 0, nCand = 0;
-    if (a < A.nLeaves)
+    if (A.items.resolve(blockIdx.x * kForceWaves + w, A.nLeaves, A.layout, a, p0, p1))
     {
         const int iLeaf = a;
-        const uint32_t p0 = A.layout[iLeaf], p1 = A.layout[iLeaf + 1];
         const uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
         const DBox box = A.boxes[A.leafSet[iLeaf]];
         const double pmass = A.pmass, beta = A.beta;
@@ -87,11 +91,16 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
             const double hsi = A.hsqinv[i], rhoi = A.rhoh[i], gki = A.grkfac[i], hinvi = A.hinv[i];
             const double pro2i = A.pro2[i], vwi = A.cs[i], ali = A.alpha[i], eni = A.u[i];
             const double rho1i = A.rho1[i], pri = A.pres[i], aui = A.auterm[i];
-            if (lane < c)
+            double cbc[3], cbs[3], rChunk;   // the chunk's bounding box and largest reach R h
             {
-                const uint32_t ic = q0 + (uint32_t)lane;
-                cp[w][0][lane] = A.x[ic]; cp[w][1][lane] = A.y[ic]; cp[w][2][lane] = A.z[ic];
-                cp[w][3][lane] = kRad * A.h[ic];
+                const uint32_t ic = q0 + (uint32_t)(lane < c ? lane : 0);
+                const double xc = A.x[ic], yc = A.y[ic], zc = A.z[ic], rc = kRad * A.h[ic];
+                if (lane < c)
+                {
+                    cp[w][0][lane] = xc; cp[w][1][lane] = yc; cp[w][2][lane] = zc;
+                    cp[w][3][lane] = rc;
+                }
+                waveBox(lane < c, xc, yc, zc, rc, cbc, cbs, rChunk);
             }
             waveLdsSync();
             double ix = 0, iy = 0, iz = 0, jx = 0, jy = 0, jz = 0, f4s = 0, vmax = 0, divs = 0;
@@ -109,6 +118,10 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
                 }
                 return false;
             };
+            auto keepJ = [&](int j) {
+                const double r = fmax(rChunk, kRad * A.h[j]);
+                return pointBoxGap2<PER>(A.x[j], A.y[j], A.z[j], cbc, cbs, box) <= r * r * kCullSlack;
+            };
             auto load = [&](int l, int j) {
                 tf[w][X][l] = A.x[j]; tf[w][Y][l] = A.y[j]; tf[w][Z][l] = A.z[j]; tf[w][H][l] = A.h[j];
                 tf[w][VX][l] = A.vx[j]; tf[w][VY][l] = A.vy[j]; tf[w][VZ][l] = A.vz[j];
@@ -117,14 +130,27 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
                 tf[w][CS][l] = A.cs[j]; tf[w][AL][l] = A.alpha[j]; tf[w][U][l] = A.u[j];
                 tf[w][PR][l] = A.pres[j]; tf[w][AU][l] = A.auterm[j];
             };
-            auto pair = [&](int k) {
-                double dx = xi - tf[w][X][k], dy = yi - tf[w][Y][k], dz = zi - tf[w][Z][k];
+            auto sep = [&](int k, double& dx, double& dy, double& dz) {
+                dx = xi - tf[w][X][k]; dy = yi - tf[w][Y][k]; dz = zi - tf[w][Z][k];
                 if constexpr (PER)
                 {
                     if (fabs(dx) > 0.5 * box.l[0]) dx = dx - box.l[0] * copysign(1.0, dx);
                     if (fabs(dy) > 0.5 * box.l[1]) dy = dy - box.l[1] * copysign(1.0, dy);
                     if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
                 }
+            };
+            // the pair test: 0 < r < R max(h_i, h_j)
+            auto accepts = [&](int k) {
+                double dx, dy, dz;
+                sep(k, dx, dy, dz);
+                const double dr2 = dx * dx + dy * dy + dz * dz;
+                const double hj = tf[w][H][k];
+                const double rmax2 = kRad2 * fmax(hi_ * hi_, hj * hj) * (1.0 + 1e-10);
+                return (dr2 > 0.0) && (dr2 < rmax2);
+            };
+            auto pair = [&](int k) {
+                double dx, dy, dz;
+                sep(k, dx, dy, dz);
                 const double dr2 = dx * dx + dy * dy + dz * dz;
                 const double hj = tf[w][H][k];
                 const double rmax2 = kRad2 * fmax(hi_ * hi_, hj * hj) * (1.0 + 1e-10);
@@ -202,12 +228,24 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
                 }
                 acc += counted ? 1 : 0;
             };
-            // lane sub of a group takes slots sub, sub + g, ... below nt (g = 1: the list order)
+            // lane sub of a group takes slots sub, sub + g, ... below nt (g = 1: the list order):
+            // the pair test of every slot into a bit mask, then the pair terms of the accepted
+            // slots in slot order, so that a wavefront runs the pair terms as often as its
+            // busiest lane accepts a pair rather than once per candidate
             auto consume = [&](int nt) {
                 if (sub == 0) cand += (uint32_t)nt;
-                for (int k = sub; k < nt; k += g) pair(k);
+                uint64_t accept = 0;
+                int t = 0;
+                for (int k = sub; k < nt; k += g, ++t)
+                    if (accepts(k)) accept |= 1ull << t;
+                while (accept)
+                {
+                    const int tt = __builtin_ctzll(accept);
+                    accept &= accept - 1;
+                    pair(sub + tt * g);
+                }
             };
-            forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, [](int) {}, consume);
+            forEachTileCompact(segStart[w], segJ[w], jq[w], A.layout, A.jList, lo, hi, keep, keepJ, load, consume);
             // group sums (and the max of vsigmax): a tree over lanes sub .. g - 1 of each group
             for (int off = 1; off < g; off <<= 1)
             {
@@ -258,15 +296,16 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixed(ForceArgs A)
     __shared__ double td[kForceWaves][ND][kWave];
     __shared__ float  ts[kForceWaves][NS][kWave];
     __shared__ double cp[kForceWaves][4][kWave];   // the chunk's particles: x, y, z, R h
+    __shared__ int    jq[kForceWaves][2 * kWave];  // candidates that passed the particle test
     const int w = threadIdx.x >> 6, lane = laneId();
-    const int a = blockIdx.x * kForceWaves + w;
     const size_t S = A.f32Stride;
     auto F = [&](int k, uint32_t idx) { return A.f32[(size_t)k * S + idx]; };
+    int a;
+    uint32_t p0, p1;
     unsigned long long nAcc = 0, nCand = 0;
-    if (a < A.nLeaves)
+    if (A.items.resolve(blockIdx.x * kForceWaves + w, A.nLeaves, A.layout, a, p0, p1))
     {
         const int iLeaf = a;
-        const uint32_t p0 = A.layout[iLeaf], p1 = A.layout[iLeaf + 1];
         const uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
         const DBox box = A.boxes[A.leafSet[iLeaf]];
         const float pm = (float)A.pmass, beta = (float)A.beta;
@@ -283,11 +322,16 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixed(ForceArgs A)
             const float hif = (float)A.h[i];
             const float hsi = F(0, i), hinvi = F(1, i), gki = F(2, i), rhoi = F(3, i), rho1i = F(4, i);
             const float pro2i = F(5, i), vwi = F(6, i), ali = F(7, i), pri = F(8, i), aui = F(9, i);
-            if (lane < c)
+            double cbc[3], cbs[3], rChunk;   // the chunk's bounding box and largest reach R h
             {
-                const uint32_t ic = q0 + (uint32_t)lane;
-                cp[w][0][lane] = A.x[ic]; cp[w][1][lane] = A.y[ic]; cp[w][2][lane] = A.z[ic];
-                cp[w][3][lane] = kRad * A.h[ic];
+                const uint32_t ic = q0 + (uint32_t)(lane < c ? lane : 0);
+                const double xc = A.x[ic], yc = A.y[ic], zc = A.z[ic], rc = kRad * A.h[ic];
+                if (lane < c)
+                {
+                    cp[w][0][lane] = xc; cp[w][1][lane] = yc; cp[w][2][lane] = zc;
+                    cp[w][3][lane] = rc;
+                }
+                waveBox(lane < c, xc, yc, zc, rc, cbc, cbs, rChunk);
             }
             waveLdsSync();
             double ix = 0, iy = 0, iz = 0, jx = 0, jy = 0, jz = 0, f4s = 0, divs = 0;
@@ -306,6 +350,10 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixed(ForceArgs A)
                 }
                 return false;
             };
+            auto keepJ = [&](int j) {
+                const double r = fmax(rChunk, kRad * A.h[j]);
+                return pointBoxGap2<PER>(A.x[j], A.y[j], A.z[j], cbc, cbs, box) <= r * r * kCullSlack;
+            };
             auto load = [&](int l, int j) {
                 td[w][X][l] = A.x[j]; td[w][Y][l] = A.y[j]; td[w][Z][l] = A.z[j];
                 td[w][VX][l] = A.vx[j]; td[w][VY][l] = A.vy[j]; td[w][VZ][l] = A.vz[j]; td[w][U][l] = A.u[j];
@@ -314,7 +362,7 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixed(ForceArgs A)
                 ts[w][R1][l] = F(4, j); ts[w][P2][l] = F(5, j); ts[w][CS][l] = F(6, j); ts[w][AL][l] = F(7, j);
                 ts[w][PR][l] = F(8, j); ts[w][AU][l] = F(9, j);
             };
-            auto pair = [&](int k) {
+            auto sep = [&](int k, float& dx, float& dy, float& dz) {
                 double dxd = xi - td[w][X][k], dyd = yi - td[w][Y][k], dzd = zi - td[w][Z][k];
                 if constexpr (PER)
                 {
@@ -322,7 +370,20 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixed(ForceArgs A)
                     if (fabs(dyd) > 0.5 * box.l[1]) dyd = dyd - box.l[1] * copysign(1.0, dyd);
                     if (fabs(dzd) > 0.5 * box.l[2]) dzd = dzd - box.l[2] * copysign(1.0, dzd);
                 }
-                const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+                dx = (float)dxd; dy = (float)dyd; dz = (float)dzd;
+            };
+            // the pair test: 0 < r < R max(h_i, h_j)
+            auto accepts = [&](int k) {
+                float dx, dy, dz;
+                sep(k, dx, dy, dz);
+                const float dr2 = dx * dx + dy * dy + dz * dz;
+                const float hj = ts[w][H][k];
+                const float rmax2 = R2 * fmaxf(hif * hif, hj * hj) * (1.0f + 1e-6f);
+                return (dr2 > 0.0f) && (dr2 < rmax2);
+            };
+            auto pair = [&](int k) {
+                float dx, dy, dz;
+                sep(k, dx, dy, dz);
                 const float dr2 = dx * dx + dy * dy + dz * dz;
                 const float hj = ts[w][H][k];
                 const float rmax2 = R2 * fmaxf(hif * hif, hj * hj) * (1.0f + 1e-6f);
@@ -406,12 +467,21 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixed(ForceArgs A)
                 f4s += (double)pf4; divs += (double)pdiv;
                 pix = piy = piz = pjx = pjy = pjz = pf4 = pdiv = 0.0f;
             };
-            auto consume = [&](int nt) {
+            auto consume = [&](int nt) {   // in two phases, as forceKernel's
                 if (sub == 0) cand += (uint32_t)nt;
-                for (int k = sub; k < nt; k += g) pair(k);
+                uint64_t accept = 0;
+                int t = 0;
+                for (int k = sub; k < nt; k += g, ++t)
+                    if (accepts(k)) accept |= 1ull << t;
+                while (accept)
+                {
+                    const int tt = __builtin_ctzll(accept);
+                    accept &= accept - 1;
+                    pair(sub + tt * g);
+                }
                 flush();
             };
-            forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, [](int) {}, consume);
+            forEachTileCompact(segStart[w], segJ[w], jq[w], A.layout, A.jList, lo, hi, keep, keepJ, load, consume);
             double vm = (double)vmax;
             for (int off = 1; off < g; off <<= 1)
             {
@@ -485,7 +555,7 @@ void launchForceF32(const ForceArgs& a, int n, float* dst, hipStream_t st)
 void launchForce(const ForceArgs& a, bool periodic, bool disc, bool full, bool mixed, hipStream_t st)
 {
     if (a.nLeaves <= 0) return;
-    const int grid = iceil(a.nLeaves, kForceWaves);
+    const int grid = iceil(a.items.count(a.nLeaves), kForceWaves);
 #define CSPH_F(P, D, F)                                                             \
     do {                                                                            \
         if (mixed) forceKernelMixed<P, D, F><<<grid, kForceThreads, 0, st>>>(a);    \

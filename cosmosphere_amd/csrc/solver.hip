@@ -351,6 +351,7 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         g.cx = s.nodeCx.as<double>(); g.cy = s.nodeCy.as<double>(); g.cz = s.nodeCz.as<double>();
         g.lbox = s.leafBox.as<double>(); g.lboxN = s.nLeaves;
         g.sx = s.nodeSx.as<double>(); g.sy = s.nodeSy.as<double>(); g.sz = s.nodeSz.as<double>();
+        g.items = buildSweepItems(s, g.nLeaves, s.sweepSplit, st);
         launchGradients(g, per, st);
         launchDeadGradients(na, n, s.divv.as<double>(), s.xi.as<double>(), s.ddivvdt.as<double>(), st);
         g2.rec(st);
@@ -432,6 +433,7 @@ ForceTimings computeForces(State& s, double pmass, double beta, double alphau, b
         launchForceF32(a, n, f32, st);
         a.f32 = f32;
     }
+    a.items = buildSweepItems(s, a.nLeaves, s.sweepSplit, st);
     launchForce(a, per, disc, pdv && shock, mixed, st);
     launchDeadForce(na, n, s.fx.as<double>(), s.fy.as<double>(), s.fz.as<double>(), s.f4.as<double>(),
                     s.vsig.as<double>(), s.divvF.as<double>(), st);

@@ -103,6 +103,7 @@ struct State
     DevBuf pmassSet;          // ensemble: particle mass of each set
     int usePmassSet = 0;
     int totalAlive = 0;
+    int sweepHeavyItems = 0;   // per-particle items of the last gradient or force sweep
 
     // ---- sort / tree scratch ----
     DevBuf keys, keysAlt, vals, valsAlt, hist, sortSetKey;
@@ -140,6 +141,7 @@ struct State
     DevBuf jCount, jOffset, jList;
     DevBuf heavyList, heavyFlag, heavyN;   // lists walked by a whole block (long walks)
     DevBuf qSub, splitPart, splitDone;     // device-driven loop: heavy items split over wavefronts
+    DevBuf sweepFlag, sweepCount, sweepOff, sweepItems;   // gradient / force sweeps: per-particle items of long lists
     int    listLeaves = 0;    // number of leaves the CSR lists cover (all-leaf lists)
     int    listTotal  = 0;
     int    listSymmetric = 0; // 1 when the lists are the force (symmetric) lists
@@ -176,6 +178,7 @@ struct State
     int listHeavySteps = 0;        // verification: walk steps before a list goes to the block walk (0: default)
     int listExpandLeaves = -1;     // verification: block walk expands nodes above this many leaves (-1: default)
     int splitLen = -1;             // device-driven loop: heavy items with longer lists are split (-1: default, 0: off)
+    int sweepSplit = 512;          // gradient / force sweeps: leaves with more list entries go one particle per wavefront (0: off)
     std::vector<PassRecord> passLog;   // one record per NR pass of the last solve
     int      closureOn = -1;   // -1: on for mixed, off for plain
     int      treeOnly = 0;     // verification: stop after the tree build

@@ -65,7 +65,133 @@ __device__ inline void forEachTileIf(int* __restrict__ segStart, int* __restrict
     }
 }
 
+// Stage, in list order, the j-particles of list entries [lo, hi) whose leaf keep(jl) accepts
+// and which keepJ(j) accepts, as full 64-slot tiles: each round every lane tests one candidate,
+// the accepted ones join a queue (LDS, 2 kWave entries), and each time 64 are queued (and once
+// more for the rest at the end) load(slot, j) runs for every slot, then consume(nt).  A
+// particle-level test that rejects most of a leaf's particles then costs no pair sweeps.
+template<class Keep, class KeepJ, class Load, class Consume>
+__device__ inline void forEachTileCompact(int* __restrict__ segStart, int* __restrict__ segJ, int* __restrict__ queue,
+                                          const uint32_t* __restrict__ layout, const int* __restrict__ jList,
+                                          uint32_t lo, uint32_t hi, Keep keep, KeepJ keepJ, Load load,
+                                          Consume consume)
+{
+    const int lane = laneId();
+    int fill = 0;   // queued entries, wave-uniform, < kWave between rounds
+    for (uint32_t e0 = lo; e0 < hi; e0 += kWave)
+    {
+        const uint32_t e = e0 + lane;
+        int js = 0, jc = 0;
+        if (e < hi)
+        {
+            const int jl = jList[e];
+            if (keep(jl))
+            {
+                js = (int)layout[jl];
+                jc = (int)layout[jl + 1] - js;
+            }
+        }
+        const int incl  = waveInclusiveScan(jc);
+        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));   // wave-uniform
+        segStart[lane] = incl - jc;
+        segJ[lane]     = js;
+        const int nseg = (int)min((uint32_t)kWave, hi - e0);
+        waveLdsSync();
+        for (int s0 = 0; s0 < total; s0 += kWave)
+        {
+            const int slot = s0 + lane;
+            int j = -1;
+            if (slot < total)
+            {
+                int a = 0, b = nseg;   // last t with segStart[t] <= slot
+                while (b - a > 1)
+                {
+                    const int m = (a + b) >> 1;
+                    if (segStart[m] <= slot) a = m;
+                    else                     b = m;
+                }
+                j = segJ[a] + slot - segStart[a];
+            }
+            const bool ok = j >= 0 && keepJ(j);
+            const uint64_t m = ballot64(ok);
+            if (ok) queue[fill + __popcll(m & lanemaskLt())] = j;   // < 2 kWave
+            fill += __popcll(m);
+            if (fill >= kWave)
+            {
+                waveLdsSync();
+                load(lane, queue[lane]);
+                const int rest = queue[kWave + lane];   // entries kWave .. fill - 1 move to the front
+                waveLdsSync();
+                consume(kWave);
+                waveLdsSync();
+                fill -= kWave;
+                if (lane < fill) queue[lane] = rest;
+            }
+        }
+        waveLdsSync();   // segStart, segJ and the queue are rewritten next
+    }
+    if (fill > 0)
+    {
+        if (lane < fill) load(lane, queue[lane]);
+        waveLdsSync();
+        consume(fill);
+        waveLdsSync();
+    }
+}
+
 constexpr double kCullSlack = 1.0 + 1e-5;
+
+// Bounding box of the points of the lanes with own set (centre c, half-sizes s rounded up so
+// that every point lies inside) and the largest r over them, in every lane.
+__device__ inline void waveBox(bool own, double x, double y, double z, double r, double c[3], double s[3],
+                               double& rmax)
+{
+    const double p[3] = {x, y, z};
+    double lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        lo[a] = own ? p[a] : __builtin_inf();
+        hi[a] = own ? p[a] : -__builtin_inf();
+    }
+    double rm = own ? r : 0.0;
+    for (int off = 1; off < kWave; off <<= 1)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+        {
+            lo[a] = fmin(lo[a], __shfl_xor(lo[a], off, kWave));
+            hi[a] = fmax(hi[a], __shfl_xor(hi[a], off, kWave));
+        }
+        rm = fmax(rm, __shfl_xor(rm, off, kWave));
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        c[a] = 0.5 * (lo[a] + hi[a]);
+        s[a] = 0.5 * (hi[a] - lo[a]) + 0x1p-50 * fmax(fabs(lo[a]), fabs(hi[a]));
+    }
+    rmax = rm;
+}
+
+// Squared gap between the point p and the box (centre c, half-sizes s; nearest periodic image of
+// the centre per axis).
+template<bool PER>
+__device__ inline double pointBoxGap2(double px, double py, double pz, const double c[3], const double s[3],
+                                      const DBox& box)
+{
+    const double p[3] = {px, py, pz};
+    double d2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        double d = c[a] - p[a];
+        if constexpr (PER) d -= box.l[a] * rint(d * box.il[a]);
+        const double v = fmax(fabs(d) - s[a], 0.0);
+        d2 += v * v;
+    }
+    return d2;
+}
 
 // Squared gap between the point p and the tight box of leaf jl's particles (lb: centres x, y, z
 // then half-sizes x, y, z, n doubles each; nearest periodic image of the centre per axis).

@@ -41,10 +41,11 @@ def _default_walk():
         lib = Library(kernel)
         lib.set_option("list_heavy_steps", 0)
         lib.set_option("list_expand_leaves", -1)
+        lib.set_option("sweep_split", 512)
 
 
 @pytest.mark.parametrize("variant", [dict(), dict(disc=1), dict(pdv=0), dict(shock=0, quintic=False),
-                                     dict(block=1)])
+                                     dict(block=1), dict(sweep=4)])
 def test_force_parity(variant, _default_walk):
     quintic = variant.get("quintic", True)
     c, v, u = setup(quintic=quintic)
@@ -57,11 +58,15 @@ def test_force_parity(variant, _default_walk):
     if variant.get("block"):   # every list (gather and symmetric) through the block walk
         lib.set_option("list_heavy_steps", 1)
         lib.set_option("list_expand_leaves", 0)
+    # leaves with more than `sweep` list entries: one wavefront per particle in the sweeps
+    lib.set_option("sweep_split", variant.get("sweep", 512))
     out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], hfact=c["hfact"], v=v)
     th = thermo(c, out["h"], u)
     lib.upload_sorted(THERMO, th)
     lib.force(n, c["pmass"], beta=2.0, alphau=1.0, disc_viscosity=disc, pdv_heating=pdv, shock_heating=shock)
     f = lib.download(FORCE_OUT, n)
+    if variant.get("sweep"):
+        assert lib.stat("sweep_items") > n // 2   # most particles went through per-particle items
 
     o = O.Oracle(quintic=c["quintic"])
     o.set_particles(c["x"], c["y"], c["z"], c["h"], v=v, a=np.zeros((3, n)))
@@ -116,7 +121,7 @@ def test_force_repeat_reuses_lists():
     np.testing.assert_array_equal(lib.download(HSML, n)[0], out["h"])
 
 
-@pytest.mark.parametrize("variant", [dict(), dict(disc=1), dict(shock=0, quintic=False)])
+@pytest.mark.parametrize("variant", [dict(), dict(disc=1), dict(shock=0, quintic=False), dict(sweep=4)])
 def test_force_mixed_vs_oracle(variant):
     """Mixed form of the force pass (FP32 pair algebra from FP64 differences, FP64 sums) against
     the FP64 oracle at the same h: accelerations, du/dt and div v within 2e-6 of their largest
@@ -129,17 +134,21 @@ def test_force_mixed_vs_oracle(variant):
     lib.set_option("precision", 1)
     lib.set_option("max_iter", 100)
     lib.set_option("tree_only", 0)
+    lib.set_option("sweep_split", variant.get("sweep", 512))
     try:
         out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], hfact=c["hfact"], v=v)
         th = thermo(c, out["h"], u)
         lib.upload_sorted(THERMO, th)
         lib.force(n, c["pmass"], beta=2.0, alphau=1.0, disc_viscosity=disc, pdv_heating=pdv, shock_heating=shock)
         f = lib.download(FORCE_OUT, n)
+        if variant.get("sweep"):
+            assert lib.stat("sweep_items") > n // 2
         h_sorted = lib.array("h")
         rho_sorted = lib.array("rho")
         gradh_sorted = lib.array("gradh")
     finally:
         lib.set_option("precision", 0)
+        lib.set_option("sweep_split", 512)
     # the oracle at the GPU's own h and d(rho)/d(h), so only the force arithmetic differs
     o = O.Oracle(quintic=c["quintic"])
     o.set_particles(c["x"], c["y"], c["z"], c["h"], v=v, a=np.zeros((3, n)))

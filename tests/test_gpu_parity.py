@@ -87,8 +87,8 @@ def test_jlists_bit_exact(name, walk):
     np.testing.assert_array_equal(lib.array("jList"), o.get("jList"))
 
 
-@pytest.mark.parametrize("name", ["lattice", "plummer_quintic"])
-def test_gradient_sweep(name):
+@pytest.mark.parametrize("name,sweep", [("lattice", 512), ("plummer_quintic", 512), ("plummer_quintic", 4)])
+def test_gradient_sweep(name, sweep):
     c = dataset(name)
     n = c["n"]
     idx = np.arange(n)
@@ -101,7 +101,14 @@ def test_gradient_sweep(name):
     lib = Library(c["kernel"])
     lib.set_option("precision", 0)
     lib.set_option("max_iter", 100)
-    out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], tolh=1e-4, hfact=c["hfact"], v=v, a=a)
+    # leaves with more than `sweep` list entries: one wavefront per particle in the sweep
+    lib.set_option("sweep_split", sweep)
+    try:
+        out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], tolh=1e-4, hfact=c["hfact"], v=v, a=a)
+        if sweep < 512:
+            assert lib.stat("sweep_items") > n // 2
+    finally:
+        lib.set_option("sweep_split", 512)
     o, _ = oracle_run(c, gradients=True, v=v, a=a)
     order = o.get("order")
 
