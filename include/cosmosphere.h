@@ -108,7 +108,8 @@ int cosmo_ext_solve_dens_h(int n, double pmass, int mode, int withGradients, int
  * walk steps per 32768 lists before a list is handed to a whole block, default 256; nodes with
  * more leaves than this are expanded into separate walks, default 256), "split_len" (device-driven
  * loop: a heavy work item whose list has more entries is split over several wavefronts, default
- * 2048; 0: never).
+ * 2048; 0: never), "sweep_split" (gradient sweep and force pass: the particles of a leaf whose
+ * list has more entries go one per wavefront, default 512; 0: never).
  * The environment variable COSMO_PRECISION=mixed|fp64 sets the precision for
  * densityiterate_gpu_c. */
 void cosmo_ext_set_option(const char* name, double value);
