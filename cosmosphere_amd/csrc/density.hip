@@ -73,8 +73,10 @@ __device__ inline double xiLimiter(const double g[9])
 // whole wavefront busy.  A list entry is staged only if its leaf's particles (their tight box)
 // lie within the support R h_i of some particle of the chunk: the reference's list reaches R hmax
 // around the whole leaf box, most of which no particle's own support touches, and a leaf the
-// test drops holds no pair with q < R, so the pair sums are unchanged (for g = 1 even their
-// order is: the list order).
+// test drops holds no pair with q < R.  Of a kept leaf, a particle is staged only if it lies
+// within the largest support of the chunk's bounding box; the survivors fill the tiles in list
+// order (forEachTileCompact), so the pair sums are unchanged (for g = 1 even their order is:
+// the list order).
 template<bool PER>
 __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4))) void gradientKernel(GradArgs A)
 {
@@ -82,6 +84,8 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
     __shared__ int    segJ[kDensWaves][kWave];
     __shared__ double tj[kDensWaves][9][kWave];
     __shared__ double cp[kDensWaves][4][kWave];   // the chunk's particles: x, y, z, (R h)^2 slack
+    __shared__ int    jq[kDensWaves][2 * kWave];  // candidates that passed the particle test
+    __shared__ double cb[kDensWaves][8];          // the chunk's box: centre, half-sizes, (R hmax)^2 slack
     const int w = threadIdx.x >> 6, lane = laneId();
     int a;
     uint32_t p0, p1;
@@ -101,13 +105,24 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
         const double vxi = A.vx[i], vyi = A.vy[i], vzi = A.vz[i];
         const double axi = A.ax[i], ayi = A.ay[i], azi = A.az[i];
         const double hi_ = A.h[i];
-        const double hsi = 1.0 / (hi_ * hi_);
-        if (lane < c)
+        const double hsi = 1.0 / (hi_ * hi_), hinvi = 1.0 / hi_;
         {
-            const uint32_t ic = q0 + (uint32_t)lane;
-            const double hc = A.h[ic];
-            cp[w][0][lane] = A.x[ic]; cp[w][1][lane] = A.y[ic]; cp[w][2][lane] = A.z[ic];
-            cp[w][3][lane] = (kRad * hc) * (kRad * hc) * kCullSlack;
+            const uint32_t ic = q0 + (uint32_t)(lane < c ? lane : 0);
+            const double xc = A.x[ic], yc = A.y[ic], zc = A.z[ic], hc = A.h[ic];
+            if (lane < c)
+            {
+                cp[w][0][lane] = xc; cp[w][1][lane] = yc; cp[w][2][lane] = zc;
+                cp[w][3][lane] = (kRad * hc) * (kRad * hc) * kCullSlack;
+            }
+            // the chunk's bounding box and largest reach, kept in LDS (registers are scarce here)
+            double c3[3], s3[3], rm;
+            waveBox(lane < c, xc, yc, zc, kRad * hc, c3, s3, rm);
+            if (lane == 0)
+            {
+                cb[w][0] = c3[0]; cb[w][1] = c3[1]; cb[w][2] = c3[2];
+                cb[w][3] = s3[0]; cb[w][4] = s3[1]; cb[w][5] = s3[2];
+                cb[w][6] = rm * rm * kCullSlack;
+            }
         }
         waveLdsSync();
         double rhoi = 0, gradhi = 0, divs = 0, dv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, da[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -117,6 +132,11 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
                 if (pointLeafGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.lbox, A.lboxN, jl, box) <= cp[w][3][k])
                     return true;
             return false;
+        };
+        // of a kept leaf, a particle only if it lies within the largest support of the chunk's box
+        auto keepJ = [&](int j) {
+            const double c3[3] = {cb[w][0], cb[w][1], cb[w][2]}, s3[3] = {cb[w][3], cb[w][4], cb[w][5]};
+            return pointBoxGap2<PER>(A.x[j], A.y[j], A.z[j], c3, s3, box) <= cb[w][6];
         };
         // all nine loads in flight before the first LDS write (one memory round trip per tile;
         // left to itself the scheduler pairs each two loads with their write)
@@ -141,12 +161,17 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
             const double r2 = dx * dx + dy * dy + dz * dz;
             const double q2 = r2 * hsi;
             if (q2 >= kRad2) return;
-            const double q = sqrt(q2);
+            // one root: q = r / h (within an ulp of sqrt(q^2)); 1 / (r + eps) by the hardware
+            // reciprocal and two Newton steps (within an ulp of the division)
+            const double rij = sqrt(r2), q = rij * hinvi;
             double wk, dwk;
             kernWdW<double>(q2, q, wk, dwk);
             rhoi += wk;
             gradhi += -q * dwk - 3.0 * wk;
-            const double rij = sqrt(r2), rij1 = 1.0 / (rij + 2.220446049250313e-16);
+            const double den = rij + 2.220446049250313e-16;
+            double rij1 = __builtin_amdgcn_rcp(den);
+            rij1 = fma(rij1, fma(-den, rij1, 1.0), rij1);
+            rij1 = fma(rij1, fma(-den, rij1, 1.0), rij1);
             const double rg = rij1 * dwk;
             const double rx = dx * rg * pmass, ry = dy * rg * pmass, rz = dz * rg * pmass;
             const double dvx = vxi - tj[w][3][k], dvy = vyi - tj[w][4][k], dvz = vzi - tj[w][5][k];
@@ -182,7 +207,7 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
                 pair(sub + tt * g);
             }
         };
-        forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, [](int) {}, consume);
+        forEachTileCompact(segStart[w], segJ[w], jq[w], A.layout, A.jList, lo, hi, keep, keepJ, load, consume);
         // group sums: a tree over lanes sub .. g - 1 of each group (lane sub = 0 ends with them)
         for (int off = 1; off < g; off <<= 1)
         {
