@@ -6,6 +6,25 @@ from cosmosphere_amd import Library, workloads
 from oracle import oracle as O
 
 
+def periodic_lattice(npartx=12):
+    """Jittered close-packed lattice filling a periodic box (cubic kernel, hfact 1.2)."""
+    x, y, z, dx = workloads.make_lattice(npartx)
+    dy = dx * np.sqrt(3.0) / 2.0
+    dz = dx * np.sqrt(6.0) / 3.0
+    ny = int(round((y.max() - (y.max() % dy)) / dy)) + 1
+    nz = int(round(z.max() / dz)) + 1
+    box = np.array([0.0, 1.0, 0.0, ny * dy, 0.0, nz * dz])
+    rng = np.random.default_rng(5)
+    x = (x + 0.1 * dx * rng.normal(size=x.size)) % 1.0
+    y = (y + 0.1 * dx * rng.normal(size=x.size)) % box[3]
+    z = (z + 0.1 * dx * rng.normal(size=x.size)) % box[5]
+    n = x.size
+    vol = box[1] * box[3] * box[5]
+    c = workloads._pack(x, y, z, np.full(n, 1.2 * (vol / n) ** (1 / 3)), "cubic", "periodic lattice")
+    c["pmass"] = 1.0 / n
+    return c, box
+
+
 def dataset(name):
     """Small versions of the benchmark inputs plus edge cases."""
     if name == "lattice":

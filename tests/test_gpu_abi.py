@@ -11,7 +11,7 @@ import pytest
 
 from cosmosphere_amd import Library, workloads
 from cosmosphere_amd.api import ACCEL, DENS_OUT, HSML, POS, THERMO, VEL
-from gpu_common import gpu_run, oracle_run, rel_err
+from gpu_common import gpu_run, oracle_run, periodic_lattice, rel_err
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -64,24 +64,6 @@ def test_warp_per_leaf_mode(name):
     na = o.geti("nAlive")
     assert rel_err(lib.array("h")[:na], o.get("h")[:na]) < 1e-10
     assert rel_err(lib.array("rho")[:na], o.get("rho")[:na]) < 1e-10
-
-
-def periodic_lattice(npartx=12):
-    x, y, z, dx = workloads.make_lattice(npartx)
-    dy = dx * np.sqrt(3.0) / 2.0
-    dz = dx * np.sqrt(6.0) / 3.0
-    ny = int(round((y.max() - (y.max() % dy)) / dy)) + 1
-    nz = int(round(z.max() / dz)) + 1
-    box = np.array([0.0, 1.0, 0.0, ny * dy, 0.0, nz * dz])
-    rng = np.random.default_rng(5)
-    x = (x + 0.1 * dx * rng.normal(size=x.size)) % 1.0
-    y = (y + 0.1 * dx * rng.normal(size=x.size)) % box[3]
-    z = (z + 0.1 * dx * rng.normal(size=x.size)) % box[5]
-    n = x.size
-    vol = box[1] * box[3] * box[5]
-    c = workloads._pack(x, y, z, np.full(n, 1.2 * (vol / n) ** (1 / 3)), "cubic", "periodic lattice")
-    c["pmass"] = 1.0 / n
-    return c, box
 
 
 # every list through the block walk and every heavy item split (the periodic instances of both)
