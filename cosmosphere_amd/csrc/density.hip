@@ -77,9 +77,15 @@ __device__ inline double xiLimiter(const double g[9])
 // within the largest support of the chunk's bounding box; the survivors fill the tiles in list
 // order (forEachTileCompact), so the pair sums are unchanged (for g = 1 even their order is:
 // the list order).
-template<bool PER>
+//
+// MIXED: pair separations, velocity and acceleration differences are FP64 differences rounded
+// once to FP32; r^2, q, the root, the kernel brackets (kernVD) and the 25 gradient sums are FP32;
+// the W and q D brackets of rho and d(rho)/d(h) are summed in FP64, with the FP32 rounding of
+// 1/h^2 taken out per particle as in the NR pass.  The 3x3 solve and the limiter are FP64.
+template<bool PER, bool MIXED>
 __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4))) void gradientKernel(GradArgs A)
 {
+    using G = std::conditional_t<MIXED, float, double>;   // the gradient sums
     __shared__ int    segStart[kDensWaves][kWave];
     __shared__ int    segJ[kDensWaves][kWave];
     __shared__ double tj[kDensWaves][9][kWave];
@@ -106,6 +112,8 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
         const double axi = A.ax[i], ayi = A.ay[i], azi = A.az[i];
         const double hi_ = A.h[i];
         const double hsi = 1.0 / (hi_ * hi_), hinvi = 1.0 / hi_;
+        const float hsr = float(hsi), hinvf = float(hinvi);
+        const double eps = (hsi - (double)hsr) / (double)hsr;   // hsr (1 + eps) = 1/h^2
         {
             const uint32_t ic = q0 + (uint32_t)(lane < c ? lane : 0);
             const double xc = A.x[ic], yc = A.y[ic], zc = A.z[ic], hc = A.h[ic];
@@ -125,8 +133,9 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
             }
         }
         waveLdsSync();
-        double rhoi = 0, gradhi = 0, divs = 0, dv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, da[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        double rxx = 0, rxy = 0, rxz = 0, ryy = 0, ryz = 0, rzz = 0;
+        double rhoi = 0, gradhi = 0;   // plain: sum W, sum -q dW/dq - 3W; mixed: sum V, sum q D
+        G divs = 0, dv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, da[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        G rxx = 0, rxy = 0, rxz = 0, ryy = 0, ryz = 0, rzz = 0;
         auto keep = [&](int jl) {
             for (int k = 0; k < c; ++k)
                 if (pointLeafGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.lbox, A.lboxN, jl, box) <= cp[w][3][k])
@@ -155,7 +164,42 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
                 if (fabs(dz) > 0.5 * box.l[2]) dz = dz - box.l[2] * copysign(1.0, dz);
             }
         };
+        auto pairMixed = [&](int k) {
+            double dxd, dyd, dzd;
+            sep(k, dxd, dyd, dzd);
+            const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float q2 = r2 * hsr;
+            if (!(q2 < float(kRad2))) return;
+            const float q = __builtin_amdgcn_sqrtf(q2);
+            float V, qD;
+            kernVD<float>(q, q2, V, qD);
+            rhoi += (double)V;
+            gradhi += (double)qD;
+            // dW/dq / r = -kGp D / r = -kGp q D / (r^2 / h); the self pair (r = 0) has dx = 0
+            const float rg = r2 > 0.0f ? float(-kGp) * qD * __builtin_amdgcn_rcpf(r2 * hinvf) : 0.0f;
+            const float pm = (float)pmass;
+            const float rx = dx * rg * pm, ry = dy * rg * pm, rz = dz * rg * pm;
+            const float dvx = (float)(vxi - tj[w][3][k]), dvy = (float)(vyi - tj[w][4][k]);
+            const float dvz = (float)(vzi - tj[w][5][k]);
+            divs += dvx * rx + dvy * ry + dvz * rz;
+            dv[0] += dvx * rx; dv[1] += dvx * ry; dv[2] += dvx * rz;
+            dv[3] += dvy * rx; dv[4] += dvy * ry; dv[5] += dvy * rz;
+            dv[6] += dvz * rx; dv[7] += dvz * ry; dv[8] += dvz * rz;
+            const float dax = (float)(axi - tj[w][6][k]), day = (float)(ayi - tj[w][7][k]);
+            const float daz = (float)(azi - tj[w][8][k]);
+            da[0] += dax * rx; da[1] += dax * ry; da[2] += dax * rz;
+            da[3] += day * rx; da[4] += day * ry; da[5] += day * rz;
+            da[6] += daz * rx; da[7] += daz * ry; da[8] += daz * rz;
+            rxx -= dx * rx; rxy -= dx * ry; rxz -= dx * rz;
+            ryy -= dy * ry; ryz -= dy * rz; rzz -= dz * rz;
+        };
         auto pair = [&](int k) {
+            if constexpr (MIXED)
+            {
+                pairMixed(k);
+                return;
+            }
             double dx, dy, dz;
             sep(k, dx, dy, dz);
             const double r2 = dx * dx + dy * dy + dz * dz;
@@ -198,7 +242,17 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
             {
                 double dx, dy, dz;
                 sep(k, dx, dy, dz);
-                if ((dx * dx + dy * dy + dz * dz) * hsi < kRad2) accept |= 1ull << t;
+                bool in;
+                if constexpr (MIXED)
+                {
+                    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+                    in = (fx * fx + fy * fy + fz * fz) * hsr < float(kRad2);
+                }
+                else
+                {
+                    in = (dx * dx + dy * dy + dz * dz) * hsi < kRad2;
+                }
+                if (in) accept |= 1ull << t;
             }
             while (accept)
             {
@@ -211,8 +265,8 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
         // group sums: a tree over lanes sub .. g - 1 of each group (lane sub = 0 ends with them)
         for (int off = 1; off < g; off <<= 1)
         {
-            auto add = [&](double& v) {
-                const double o = __shfl_down(v, off, kWave);
+            auto add = [&](auto& v) {
+                const auto o = __shfl_down(v, off, kWave);
                 if (sub + off < g) v += o;
             };
             add(rhoi); add(gradhi); add(divs);
@@ -223,8 +277,16 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
         if (!has || sub != 0) continue;
 
         const double hi1 = 1.0 / hi_, hi31 = hi1 * hi1 * hi1, hi41 = hi31 * hi1;
-        const double rho_i = rhoi * kCnormk * pmass * hi31;
-        const double grad_i = gradhi * kCnormk * pmass * hi41;
+        double wsum = rhoi, gsum = gradhi;
+        if constexpr (MIXED)
+        {
+            // brackets to sums, the q of every pair short by (1 + eps/2): as particleSums in nr.hip
+            const double v = rhoi - (0.5 * eps) * (kGp / kWw) * gradhi;
+            wsum = kWw * v;
+            gsum = kGp * gradhi - kGw * v;
+        }
+        const double rho_i = wsum * kCnormk * pmass * hi31;
+        const double grad_i = gsum * kCnormk * pmass * hi41;
         A.rho[i] = rho_i;
         A.gradh[i] = grad_i;
         if (!(rho_i > 0.0))
@@ -234,11 +296,16 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
         }
         const double omega = 1.0 + (hi_ / (3.0 * rho_i)) * grad_i;
         const double oinv = (omega > 0.0) ? (1.0 / omega) : 1.0;
+        // the sums in FP64 from here on (the mixed form accumulates them in FP32)
+        const double Rxx = rxx, Rxy = rxy, Rxz = rxz, Ryy = ryy, Ryz = ryz, Rzz = rzz, Divs = divs;
+        double Dv[9], Da[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) { Dv[f] = dv[f]; Da[f] = da[f]; }
         const double term = kCnormk * oinv * hi41 / rho_i;
-        const double denom = rxx * ryy * rzz + 2.0 * rxy * rxz * ryz - rxx * ryz * ryz - ryy * rxz * rxz - rzz * rxy * rxy;
-        const double rm0 = ryy * rzz - ryz * ryz, rm1 = rxz * ryz - rzz * rxy, rm2 = rxy * ryz - rxz * ryy;
-        const double rm3 = rzz * rxx - rxz * rxz, rm4 = rxy * rxz - rxx * ryz, rm5 = rxx * ryy - rxy * rxy;
-        A.divv[i] = -divs * term;
+        const double denom = Rxx * Ryy * Rzz + 2.0 * Rxy * Rxz * Ryz - Rxx * Ryz * Ryz - Ryy * Rxz * Rxz - Rzz * Rxy * Rxy;
+        const double rm0 = Ryy * Rzz - Ryz * Ryz, rm1 = Rxz * Ryz - Rzz * Rxy, rm2 = Rxy * Ryz - Rxz * Ryy;
+        const double rm3 = Rzz * Rxx - Rxz * Rxz, rm4 = Rxy * Rxz - Rxx * Ryz, rm5 = Rxx * Ryy - Rxy * Rxy;
+        A.divv[i] = -Divs * term;
         double gm[9], div_a;
         if (fabs(denom) > 2.2250738585072014e-308)
         {
@@ -246,7 +313,7 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
             for (int row = 0; row < 3; ++row)
             {
-                const double sx = dv[3 * row], sy = dv[3 * row + 1], sz = dv[3 * row + 2];
+                const double sx = Dv[3 * row], sy = Dv[3 * row + 1], sz = Dv[3 * row + 2];
                 gm[3 * row]     = -(sx * rm0 + sy * rm1 + sz * rm2) * dd;
                 gm[3 * row + 1] = -(sx * rm1 + sy * rm3 + sz * rm4) * dd;
                 gm[3 * row + 2] = -(sx * rm2 + sy * rm4 + sz * rm5) * dd;
@@ -255,7 +322,7 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
 #pragma unroll
             for (int row = 0; row < 3; ++row)
             {
-                const double sx = da[3 * row], sy = da[3 * row + 1], sz = da[3 * row + 2];
+                const double sx = Da[3 * row], sy = Da[3 * row + 1], sz = Da[3 * row + 2];
                 const double gax = (sx * rm0 + sy * rm1 + sz * rm2) * dd;
                 const double gay = (sx * rm1 + sy * rm3 + sz * rm4) * dd;
                 const double gaz = (sx * rm2 + sy * rm4 + sz * rm5) * dd;
@@ -266,8 +333,8 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
         else
         {
 #pragma unroll
-            for (int f = 0; f < 9; ++f) gm[f] = -dv[f] * term;
-            div_a = -term * (da[0] + da[4] + da[8]);
+            for (int f = 0; f < 9; ++f) gm[f] = -Dv[f] * term;
+            div_a = -term * (Da[0] + Da[4] + Da[8]);
         }
         A.xi[i] = xiLimiter(gm);
         A.ddivvdt[i] =
@@ -286,13 +353,21 @@ __global__ void deadGradientKernel(int first, int n, double* __restrict__ divv, 
 // ---------------------------------------------------------------------------------------
 // Host wrappers
 // ---------------------------------------------------------------------------------------
-void launchGradients(const GradArgs& a, bool periodic, hipStream_t st)
+void launchGradients(const GradArgs& a, bool periodic, bool mixed, hipStream_t st)
 {
     if (a.nLeaves > 0)
     {
         const int grid = iceil(a.items.count(a.nLeaves), kDensWaves);
-        if (periodic) gradientKernel<true><<<grid, kDensThreads, 0, st>>>(a);
-        else          gradientKernel<false><<<grid, kDensThreads, 0, st>>>(a);
+        if (mixed)
+        {
+            if (periodic) gradientKernel<true, true><<<grid, kDensThreads, 0, st>>>(a);
+            else          gradientKernel<false, true><<<grid, kDensThreads, 0, st>>>(a);
+        }
+        else
+        {
+            if (periodic) gradientKernel<true, false><<<grid, kDensThreads, 0, st>>>(a);
+            else          gradientKernel<false, false><<<grid, kDensThreads, 0, st>>>(a);
+        }
         LAUNCH_CHECK();
     }
 }

@@ -190,7 +190,7 @@ void launchDensity(const DensArgs& a, int precision, int mode, bool periodic, hi
 // one ctl->perParticle does not select exits at once.
 void launchDensityPersistent(const DensArgs& a, int precision, bool periodic, int blocks, hipStream_t st);
 int  densityPersistentBlocks(int precision, bool periodic);
-void launchGradients(const GradArgs& a, bool periodic, hipStream_t st);
+void launchGradients(const GradArgs& a, bool periodic, bool mixed, hipStream_t st);
 // Per-particle items of the leaves with more than `entries` list entries (0: none); reads the
 // item count back (host-synchronising).
 SweepItems buildSweepItems(State& s, int nLeaves, int entries, hipStream_t st);

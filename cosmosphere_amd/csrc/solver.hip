@@ -352,7 +352,7 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         g.lbox = s.leafBox.as<double>(); g.lboxN = s.nLeaves;
         g.sx = s.nodeSx.as<double>(); g.sy = s.nodeSy.as<double>(); g.sz = s.nodeSz.as<double>();
         g.items = buildSweepItems(s, g.nLeaves, s.sweepSplit, st);
-        launchGradients(g, per, st);
+        launchGradients(g, per, s.precision == kMixed, st);
         launchDeadGradients(na, n, s.divv.as<double>(), s.xi.as<double>(), s.ddivvdt.as<double>(), st);
         g2.rec(st);
         HIP_CHECK(hipEventSynchronize(g2.e));

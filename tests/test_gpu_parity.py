@@ -87,8 +87,13 @@ def test_jlists_bit_exact(name, walk):
     np.testing.assert_array_equal(lib.array("jList"), o.get("jList"))
 
 
-@pytest.mark.parametrize("name,sweep", [("lattice", 512), ("plummer_quintic", 512), ("plummer_quintic", 4)])
-def test_gradient_sweep(name, sweep):
+@pytest.mark.parametrize("name,sweep,precision", [("lattice", 512, 0), ("plummer_quintic", 512, 0),
+                                                  ("plummer_quintic", 4, 0), ("lattice", 512, 1),
+                                                  ("plummer_quintic", 512, 1), ("plummer_quintic", 4, 1)])
+def test_gradient_sweep(name, sweep, precision):
+    """Gradient sweep against the FP64 oracle: plain FP64 within 1e-10 (h, rho) and 1e-7 (div v,
+    d(div v)/dt); the mixed form (FP32 pair terms, FP64 rho sums) within 1e-6 (h, rho) and 2e-5 of
+    the largest |div v|, |d(div v)/dt| (xi within 2e-5)."""
     c = dataset(name)
     n = c["n"]
     idx = np.arange(n)
@@ -99,7 +104,7 @@ def test_gradient_sweep(name, sweep):
     lib = None
     from cosmosphere_amd import Library
     lib = Library(c["kernel"])
-    lib.set_option("precision", 0)
+    lib.set_option("precision", precision)
     lib.set_option("max_iter", 100)
     # leaves with more than `sweep` list entries: one wavefront per particle in the sweep
     lib.set_option("sweep_split", sweep)
@@ -109,6 +114,7 @@ def test_gradient_sweep(name, sweep):
             assert lib.stat("sweep_items") > n // 2
     finally:
         lib.set_option("sweep_split", 512)
+        lib.set_option("precision", 0)
     o, _ = oracle_run(c, gradients=True, v=v, a=a)
     order = o.get("order")
 
@@ -116,12 +122,22 @@ def test_gradient_sweep(name, sweep):
         arr = np.empty(n)
         arr[order] = o.get(name_)
         return arr
-    assert rel_err(out["h"], host("h")) < 1e-10
-    assert rel_err(out["rho"], host("rho")) < 1e-10
-    for f in ("divv", "ddivvdt"):
-        ref = host(f)
-        assert rel_err(out[f], ref, floor=np.abs(ref).max() * 1e-6) < 1e-7, f
-    assert np.max(np.abs(out["xi"] - host("xi"))) < 1e-6
+    if precision == 0:
+        assert rel_err(out["h"], host("h")) < 1e-10
+        assert rel_err(out["rho"], host("rho")) < 1e-10
+        for f in ("divv", "ddivvdt"):
+            ref = host(f)
+            assert rel_err(out[f], ref, floor=np.abs(ref).max() * 1e-6) < 1e-7, f
+        assert np.max(np.abs(out["xi"] - host("xi"))) < 1e-6
+    else:
+        errs = {"h": rel_err(out["h"], host("h")), "rho": rel_err(out["rho"], host("rho"))}
+        for f in ("divv", "ddivvdt"):
+            ref = host(f)
+            errs[f] = np.max(np.abs(out[f] - ref)) / np.abs(ref).max()
+        errs["xi"] = np.max(np.abs(out["xi"] - host("xi")))
+        print("mixed gradient sweep", name, errs, flush=True)
+        assert errs["h"] < 1e-6 and errs["rho"] < 1e-6
+        assert errs["divv"] < 2e-5 and errs["ddivvdt"] < 2e-5 and errs["xi"] < 2e-5
 
 
 @pytest.mark.parametrize("name", ["lattice", "C1small", "plummer_cubic", "plummer_quintic"])
