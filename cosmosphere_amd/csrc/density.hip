@@ -4,11 +4,12 @@
  *
  * Restates src/density_base.cu:479-714 (gradient sweep: rho and d(rho)/d(h) at the final h,
  * div v, the velocity-gradient matrix through the inverse of the r-tensor, the xi limiter and
- * d(div v)/dt) for gfx950, in FP64 as the reference.
+ * d(div v)/dt) for gfx950: in FP64 as the reference, or in the mixed form (FP32 pair terms,
+ * FP64 rho sums; gradientKernel<..., true>).
  *
  * Mapping: one wavefront per leaf, its particles up to 64 at a time, g = floor(64 / c) lanes
  * per particle (below); the particles of a leaf with a long list one wavefront each
- * (SweepItems).    The leaf's neighbour-leaf list is consumed 64 entries at a time: each
+ * (SweepItems).  The leaf's neighbour-leaf list is consumed 64 entries at a time: each
  * lane reads one j-leaf's particle range, a wave scan of the range lengths lays the
  * j-particles out as consecutive slots, and the slots are staged into an LDS tile with one
  * coalesced load per lane (j-leaf particles are contiguous in Hilbert order).  Every lane then
