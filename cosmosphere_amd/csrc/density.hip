@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
             rhoi += wk;
             gradhi += -q * dwk - 3.0 * wk;
             const double den = rij + 2.220446049250313e-16;
-            double rij1 = __builtin_amdgcn_rcp(den);
+            double rij1 = __builtin_amdgcn_rcp(den);   // as recip() in force.hip
             rij1 = fma(rij1, fma(-den, rij1, 1.0), rij1);
             rij1 = fma(rij1, fma(-den, rij1, 1.0), rij1);
             const double rg = rij1 * dwk;
