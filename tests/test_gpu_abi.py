@@ -80,30 +80,38 @@ def _default_paths():
             lib.set_option(k, val)
 
 
-@pytest.mark.parametrize("paths", ["default", "forced"])
+@pytest.mark.parametrize("paths", ["default", "forced", "mixed"])
 def test_periodic_box(paths, _default_paths):
+    """Periodic solve with gradients against the oracle: FP64 (default and forced block walks and
+    split items) within 1e-10 (h, rho) and 1e-7 of the largest |div v|, lists bit-exact; the mixed
+    form within 1e-6 (h, rho) and 2e-5 of the largest |div v|."""
     c, box = periodic_lattice()
     n = c["n"]
     rng = np.random.default_rng(6)
     v = rng.normal(size=(3, n)) * 0.1
     lib = Library("cubic")
-    lib.set_option("precision", 0)
+    lib.set_option("precision", 1 if paths == "mixed" else 0)
     for k, val in (FORCED_PATHS if paths == "forced" else DEFAULT_PATHS).items():
         lib.set_option(k, val)
-    out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], hfact=1.2, v=v, periodic_box=box)
+    try:
+        out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], hfact=1.2, v=v, periodic_box=box)
+    finally:
+        lib.set_option("precision", 0)
     o = O.Oracle(quintic=False)
     o.set_particles(c["x"], c["y"], c["z"], c["h"], v=v, a=np.zeros((3, n)))
     o.build(periodic_box=box)
     assert o.density_solve(c["pmass"], 1e-4, 1.2, 100, 0, True) == 0
     order = o.get("order")
+    tol = 1e-6 if paths == "mixed" else 1e-10
     for name in ("h", "rho"):
         ref = np.empty(n)
         ref[order] = o.get(name)
-        assert rel_err(out[name], ref) < 1e-10, name
+        assert rel_err(out[name], ref) < tol, name
     ref = np.empty(n)
     ref[order] = o.get("divv")
-    assert np.max(np.abs(out["divv"] - ref)) < 1e-7 * np.abs(ref).max()
-    np.testing.assert_array_equal(lib.array("jList"), o.get("jList"))
+    assert np.max(np.abs(out["divv"] - ref)) < (2e-5 if paths == "mixed" else 1e-7) * np.abs(ref).max()
+    if paths != "mixed":
+        np.testing.assert_array_equal(lib.array("jList"), o.get("jList"))
 
 
 def _run(snippet, env=None):
