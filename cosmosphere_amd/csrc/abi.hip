@@ -10,8 +10,13 @@
 #include "solver.hpp"
 #include "tree.hpp"
 #include "tune.hpp"
+#include "xfer.hpp"
 
+#include <sys/resource.h>
+
+#include <algorithm>
 #include <chrono>
+#include <cstdarg>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -39,117 +44,65 @@ void requireSized(State& s, int n, const char* what)
     }
 }
 
-// Slot -> device arrays, in the host's component order (include/arrays.hpp:15-25).
-std::vector<DevBuf*> slotComponents(State& s, int slot)
+// Transfers check their arguments before any device work and abort with the reference's
+// messages (Phantom's logs are read against them): size, slot number, ordering.
+[[noreturn]] void fatalf(const char* fmt, ...)
 {
-    switch (slot)
-    {
-        case COSMO_POS:       return {&s.x, &s.y, &s.z};
-        case COSMO_HSML:      return {&s.h};
-        case COSMO_VEL:       return {&s.vx, &s.vy, &s.vz};
-        case COSMO_ACCEL:     return {&s.ax, &s.ay, &s.az};
-        case COSMO_DENS_OUT:  return {&s.rho, &s.gradh};
-        case COSMO_GRAD_OUT:  return {&s.divv, &s.xi, &s.ddivvdt};
-        case COSMO_THERMO:    return {&s.pro2, &s.cs, &s.alpha, &s.u};
-        case COSMO_FORCE_OUT: return {&s.fx, &s.fy, &s.fz, &s.f4, &s.vsig, &s.divvF};
-        default:              return {};
-    }
+    std::fputs("FATAL: ", stderr);
+    va_list ap;
+    va_start(ap, fmt);
+    std::vfprintf(stderr, fmt, ap);
+    va_end(ap);
+    std::fputc('\n', stderr);
+    std::abort();
 }
 
-int slotCount(int slot)
+const SlotDesc& slotFor(const State& s, int slot, int n, const char* what, bool needOrder)
 {
-    static const int counts[9] = {0, 3, 1, 3, 3, 2, 3, 4, 6};
-    return (slot >= 1 && slot <= 8) ? counts[slot] : 0;
+    if (s.sizedFor != n) fatalf("%s(slot=%d) with n=%d but the device arrays are sized for %d", what, slot, n, s.sizedFor);
+    const SlotDesc* d = slotDesc(slot);
+    if (!d) fatalf("%s called for slot %d, which has no arrays", what, slot);
+    if (needOrder && !(s.haveOrder && s.n == n))
+        fatalf("%s(slot=%d) with no ordering: a density solve must have run first", what, slot);
+    return *d;
 }
 
-std::vector<DevBuf*> checkedSlot(State& s, int slot, int n, const char* what)
-{
-    if (s.sizedFor != n)
-    {
-        std::fprintf(stderr, "FATAL: %s(slot=%d) with n=%d but the device arrays are sized for %d\n", what, slot, n,
-                     s.sizedFor);
-        std::abort();
-    }
-    auto comps = slotComponents(s, slot);
-    if (comps.empty())
-    {
-        std::fprintf(stderr, "FATAL: %s called for slot %d, which has no arrays\n", what, slot);
-        std::abort();
-    }
-    return comps;
-}
+using Clock = std::chrono::steady_clock;
 
-void requireOrder(State& s, int slot, int n, const char* what)
-{
-    if (!s.haveOrder || s.n != n)
-    {
-        std::fprintf(stderr, "FATAL: %s(slot=%d) with no ordering: a density solve must have run first\n", what, slot);
-        std::abort();
-    }
-}
+double secondsSince(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
-__global__ void gatherKernel(const double* __restrict__ src, const int* __restrict__ order, double* __restrict__ dst,
-                             int n)
+// Input slots land in the resident order when the last solve was this set; everything else
+// in the caller's order (it is then sorted by the solve).
+void upload(int slot, const double* host, int n, bool sortedCall)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[order[i]];
-}
-
-__global__ void scatterKernel(const double* __restrict__ src, const int* __restrict__ order, double* __restrict__ dst,
-                              int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[order[i]] = src[i];
-}
-
-struct XferTimer
-{
-    double&    sink;
-    hipEvent_t a, b;
-    hipStream_t st;
-    explicit XferTimer(double& c, hipStream_t s) : sink(c), st(s)
-    {
-        HIP_CHECK(hipEventCreate(&a));
-        HIP_CHECK(hipEventCreate(&b));
-        HIP_CHECK(hipEventRecord(a, st));
-    }
-    ~XferTimer()
-    {
-        HIP_CHECK(hipEventRecord(b, st));
-        HIP_CHECK(hipEventSynchronize(b));
-        float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, a, b));
-        sink += ms * 1e-3;
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-    }
-};
-
-void uploadSlot(int slot, const double* host, int n, bool sorted, const char* what)
-{
-    State& s = state();
     if (n <= 0) return;
-    auto comps = checkedSlot(s, slot, n, what);
-    if (sorted) requireOrder(s, slot, n, what);
+    State& s = state();
+    const SlotDesc& d = slotFor(s, slot, n, sortedCall ? "cosmo_upload_sorted" : "cosmo_upload", sortedCall);
     ensureStream(s);
-    XferTimer timer(s.uploadSeconds, s.stream);
-    const size_t nb = (size_t)n * sizeof(double);
-    for (size_t c = 0; c < comps.size(); ++c)
-    {
-        const double* hc = host + c * (size_t)n;
-        if (!sorted)
-        {
-            HIP_CHECK(hipMemcpyAsync(comps[c]->as<double>(), hc, nb, hipMemcpyHostToDevice, s.stream));
-        }
-        else
-        {
-            HIP_CHECK(hipMemcpyAsync(s.stage.as<double>(), hc, nb, hipMemcpyHostToDevice, s.stream));
-            gatherKernel<<<iceil(n, 256), 256, 0, s.stream>>>(s.stage.as<double>(), s.order.as<int>(),
-                                                              comps[c]->as<double>(), n);
-            LAUNCH_CHECK();
-        }
-        HIP_CHECK(hipStreamSynchronize(s.stream));
-    }
+    const auto t0 = Clock::now();
+    const bool resident = !sortedCall && d.firstInput >= 0 && residentEligible(s, n);
+    transferIn(s, d, host, n, sortedCall || resident);
+    s.uploadTotal += secondsSince(t0);
+    // a sorted upload is read as caller order by a later solve that finds it not uploaded
+    // again (the reference's convention); a resident one as the current order
+    for (int c = 0; d.firstInput >= 0 && c < d.ncomp; ++c) s.inTag[d.firstInput + c] = resident ? kTagAligned : kTagCaller;
+    s.uploadSeconds += secondsSince(t0);
+}
+
+// COSMO_MEM: device memory in use after the first solve (hipMemGetInfo) and the process's
+// peak resident set (getrusage, KiB), once per process.
+void memoryReport(int n)
+{
+    static bool reported = false;
+    if (reported || !std::getenv("COSMO_MEM")) return;
+    reported = true;
+    size_t freeB = 0, totalB = 0;
+    HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+    rusage ru{};
+    getrusage(RUSAGE_SELF, &ru);
+    const double used = double(totalB - freeB);
+    std::fprintf(stderr, "COSMO_MEM n=%d device_used=%.2f GB of %.2f GB (%.0f bytes/particle) host_peak=%.2f GB\n",
+                 n, used / 1e9, totalB / 1e9, used / std::max(n, 1), double(ru.ru_maxrss) / 1e6);
 }
 
 void setSingle(State& s, int n)
@@ -184,48 +137,34 @@ void applyEnvPrecision(State& s)
 // =========================================================================================
 extern "C" void densityiterate_gpu_c(int n, double pmass, int periodic, const double* box, double tolh, double hfact)
 {
+    const auto tEntry = Clock::now();
     State& s = state();
     requireSized(s, n, "densityiterate_gpu_c");
     ensureStream(s);
     applyEnvPrecision(s);
-    using clk = std::chrono::steady_clock;
-    const auto t0 = clk::now();
     setSingle(s, n);
-    Timings t = solveDensH(s, pmass, 0, true, periodic ? box : nullptr, tolh, hfact);
-    const auto t2 = clk::now();
-
-    static bool memReported = false;
-    if (!memReported && std::getenv("COSMO_MEM"))
-    {
-        memReported = true;
-        size_t freeB = 0, totalB = 0;
-        HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
-        long hwmKB = 0;
-        if (FILE* fp = std::fopen("/proc/self/status", "r"))
-        {
-            char line[256];
-            while (std::fgets(line, sizeof line, fp))
-                if (std::strncmp(line, "VmHWM:", 6) == 0) { std::sscanf(line + 6, "%ld", &hwmKB); break; }
-            std::fclose(fp);
-        }
-        std::fprintf(stderr, "COSMO_MEM n=%d device_used=%.2f GB of %.2f GB (%.0f bytes/particle) host_peak=%.2f GB\n",
-                     n, (totalB - freeB) / 1e9, totalB / 1e9, double(totalB - freeB) / std::max(n, 1), hwmKB / 1e6);
-    }
+    const double vecin = secondsSince(tEntry);   // host work before the solve
+    const auto t0 = Clock::now();
+    const Timings t = solveDensH(s, pmass, 0, true, periodic ? box : nullptr, tolh, hfact);
+    const double solve = secondsSince(t0);
+    memoryReport(n);
     if (statsEnabled())
     {
-        const double solve = std::chrono::duration<double, std::milli>(t2 - t0).count();
-        const double gpu = 1e3 * (t.bbox + t.keys + t.sort + t.permute + t.tree + t.nodes + t.jbuild + t.nrkern +
-                                  t.compact + t.gjbuild + t.gradkern);
+        // device phases from the solve's events; upload / download are the host-timed
+        // transfers since the previous stats line (they sit outside the solve)
+        const double gpu = t.bbox + t.keys + t.sort + t.permute + t.tree + t.nodes + t.jbuild + t.nrkern + t.compact +
+                           t.gjbuild + t.gradkern;
         std::fprintf(stderr,
                      "COSMO_STATS n=%d leaves=%d iters=%d | vecin=%.2f upload=%.2f bbox=%.2f "
                      "keysort=%.2f tree=%.2f nodes=%.2f jbuild=%.2f nrkern=%.2f gjbuild=%.2f "
                      "gradkern=%.2f download=%.2f | gpusum=%.2f solve=%.2f unaccounted=%.2f total=%.2f\n",
-                     n, t.nLeaves, t.iters, 0.0, 1e3 * s.uploadSeconds, 1e3 * t.bbox, 1e3 * (t.keys + t.sort + t.permute),
-                     1e3 * t.tree, 1e3 * t.nodes, 1e3 * t.jbuild, 1e3 * (t.nrkern + t.compact), 1e3 * t.gjbuild,
-                     1e3 * t.gradkern, 1e3 * s.downloadSeconds, gpu, solve, solve - gpu, solve);
+                     n, t.nLeaves, t.iters, 1e3 * vecin, 1e3 * s.uploadSeconds, 1e3 * t.bbox,
+                     1e3 * (t.keys + t.sort + t.permute), 1e3 * t.tree, 1e3 * t.nodes, 1e3 * t.jbuild,
+                     1e3 * (t.nrkern + t.compact), 1e3 * t.gjbuild, 1e3 * t.gradkern, 1e3 * s.downloadSeconds,
+                     1e3 * gpu, 1e3 * solve, 1e3 * (solve - gpu), 1e3 * secondsSince(tEntry));
+        s.uploadSeconds = 0.0;
+        s.downloadSeconds = 0.0;
     }
-    s.uploadSeconds = 0.0;
-    s.downloadSeconds = 0.0;
 }
 
 extern "C" double cosmo_kernel_radius(void) { return kRad; }
@@ -233,8 +172,7 @@ extern "C" double cosmo_kernel_radius(void) { return kRad; }
 extern "C" void force_gpu_c(int n, double pmass, double beta, double alphau, int disc_viscosity, int pdv_heating,
                             int shock_heating)
 {
-    using clk = std::chrono::steady_clock;
-    const auto t0 = clk::now();
+    const auto t0 = Clock::now();
     State& s = state();
     requireSized(s, n, "force_gpu_c");
     if (!s.readyForForce(n))
@@ -246,15 +184,21 @@ extern "C" void force_gpu_c(int n, double pmass, double beta, double alphau, int
         std::abort();
     }
     ensureStream(s);
-    ForceTimings ft = computeForces(s, pmass, beta, alphau, disc_viscosity != 0, pdv_heating > 0, shock_heating > 0);
+    const ForceTimings ft =
+        computeForces(s, pmass, beta, alphau, disc_viscosity != 0, pdv_heating > 0, shock_heating > 0);
     if (statsEnabled())
     {
-        const double wall = std::chrono::duration<double>(clk::now() - t0).count();
+        // upload / download: host-timed transfers since the previous stats line (the THERMO
+        // upload before this pass, the downloads after the density solve)
+        const double wall = secondsSince(t0);
+        const double gpu = ft.upsweep + ft.jbuild + ft.prep + ft.kernel;
         std::fprintf(stderr,
                      "COSMO_FORCE n=%d leaves=%d | upsweep=%.2f jbuild=%.2f upload=%.2f prep=%.2f kernel=%.2f "
                      "download=%.2f | gpusum=%.2f wall=%.2f unaccounted=%.2f\n",
-                     s.n, s.nLeaves, 1e3 * ft.upsweep, 1e3 * ft.jbuild, 0.0, 1e3 * ft.prep, 1e3 * ft.kernel, 0.0,
-                     1e3 * ft.total, 1e3 * wall, 1e3 * (wall - ft.total));
+                     s.n, s.nLeaves, 1e3 * ft.upsweep, 1e3 * ft.jbuild, 1e3 * s.uploadSeconds, 1e3 * ft.prep,
+                     1e3 * ft.kernel, 1e3 * s.downloadSeconds, 1e3 * gpu, 1e3 * wall, 1e3 * (wall - gpu));
+        s.uploadSeconds = 0.0;
+        s.downloadSeconds = 0.0;
     }
 }
 
@@ -264,32 +208,26 @@ extern "C" void cosmo_arrays_init(int n)
     sizeParticleArrays(state(), n);
 }
 
-extern "C" int cosmo_slot_ncomp(int slot) { return slotCount(slot); }
-
-extern "C" void cosmo_upload(int slot, const double* host, int n) { uploadSlot(slot, host, n, false, "cosmo_upload"); }
-
-extern "C" void cosmo_upload_sorted(int slot, const double* host, int n)
+extern "C" int cosmo_slot_ncomp(int slot)
 {
-    uploadSlot(slot, host, n, true, "cosmo_upload_sorted");
+    const SlotDesc* d = slotDesc(slot);
+    return d ? d->ncomp : 0;
 }
+
+extern "C" void cosmo_upload(int slot, const double* host, int n) { upload(slot, host, n, false); }
+
+extern "C" void cosmo_upload_sorted(int slot, const double* host, int n) { upload(slot, host, n, true); }
 
 extern "C" void cosmo_download(int slot, double* host, int n)
 {
-    State& s = state();
     if (n <= 0) return;
-    auto comps = checkedSlot(s, slot, n, "cosmo_download");
-    requireOrder(s, slot, n, "cosmo_download");
+    State& s = state();
+    const SlotDesc& d = slotFor(s, slot, n, "cosmo_download", true);
     ensureStream(s);
-    XferTimer timer(s.downloadSeconds, s.stream);
-    const size_t nb = (size_t)n * sizeof(double);
-    for (size_t c = 0; c < comps.size(); ++c)
-    {
-        scatterKernel<<<iceil(n, 256), 256, 0, s.stream>>>(comps[c]->as<double>(), s.order.as<int>(),
-                                                           s.stage.as<double>(), n);
-        LAUNCH_CHECK();
-        HIP_CHECK(hipMemcpyAsync(host + c * (size_t)n, s.stage.as<double>(), nb, hipMemcpyDeviceToHost, s.stream));
-        HIP_CHECK(hipStreamSynchronize(s.stream));
-    }
+    const auto t0 = Clock::now();
+    transferOut(s, d, host, n);
+    s.downloadSeconds += secondsSince(t0);
+    s.downloadTotal += secondsSince(t0);
 }
 
 // Page-locking is best effort in the reference (pin_c_api.cu:22-34: failure is cleared and
@@ -346,6 +284,8 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "split_len") s.splitLen = (int)value;
     else if (nm == "sweep_split") s.sweepSplit = std::max(0, (int)value);
     else if (nm == "list_expand_leaves") s.listExpandLeaves = (int)value;
+    else if (nm == "resident") s.resident = (int)value;
+    else if (nm == "xfer_threads") s.xferThreads = std::max(1, (int)value);
     else
     {
         std::fprintf(stderr, "FATAL: cosmo_ext_set_option: unknown option '%s'\n", name);
@@ -410,6 +350,10 @@ extern "C" double cosmo_ext_get_stat(const char* name)
     if (nm == "loop_blocks_scale") return s.loopBlockScale;
     if (nm == "sweep_split") return s.sweepSplit;
     if (nm == "sweep_items") return s.sweepHeavyItems;
+    if (nm == "resident") return s.resident;
+    if (nm == "resort_movers") return t.resortMovers;
+    if (nm == "upload_s") return s.uploadTotal;
+    if (nm == "download_s") return s.downloadTotal;
     return -1.0;
 }
 
@@ -469,18 +413,22 @@ extern "C" long long cosmo_ext_get_array(const char* name, void* dst, long long 
     return count;
 }
 
-// Benchmark support: keep a device copy of POS|HSML|VEL|ACCEL as uploaded (caller order)
-// and put it back before each repeated solve, so every solve starts from the same inputs.
+// Benchmark support: keep a device copy of POS|HSML|VEL|ACCEL in the caller's order and put
+// it back before each repeated solve, so every solve starts from the same inputs.
 extern "C" void cosmo_ext_stash_inputs(void)
 {
     State& s = state();
     ensureStream(s);
+    inputsToCallerOrder(s);
     const int n = s.sizedFor;
     double* st = s.stash.get<double>((size_t)10 * n + 1);
-    DevBuf* f[10] = {&s.x, &s.y, &s.z, &s.h, &s.vx, &s.vy, &s.vz, &s.ax, &s.ay, &s.az};
-    for (int k = 0; k < 10; ++k)
-        HIP_CHECK(hipMemcpyAsync(st + (size_t)k * n, f[k]->as<double>(), (size_t)n * 8, hipMemcpyDeviceToDevice,
-                                 s.stream));
+    for (int slot = COSMO_POS; slot <= COSMO_ACCEL; ++slot)
+    {
+        const SlotDesc& d = *slotDesc(slot);
+        for (int c = 0; c < d.ncomp; ++c)
+            HIP_CHECK(hipMemcpyAsync(st + (size_t)(d.firstInput + c) * n, (s.*d.comp[c]).as<double>(), (size_t)n * 8,
+                                     hipMemcpyDeviceToDevice, s.stream));
+    }
     HIP_CHECK(hipStreamSynchronize(s.stream));
     s.stashN = n;
 }
@@ -490,12 +438,16 @@ extern "C" void cosmo_ext_restore_inputs(void)
     State& s = state();
     const int n = s.stashN;
     if (n != s.sizedFor) fatal("cosmo_ext_restore_inputs: no stash for the current particle count");
-    double* st = s.stash.as<double>();
-    DevBuf* f[10] = {&s.x, &s.y, &s.z, &s.h, &s.vx, &s.vy, &s.vz, &s.ax, &s.ay, &s.az};
-    for (int k = 0; k < 10; ++k)
-        HIP_CHECK(hipMemcpyAsync(f[k]->as<double>(), st + (size_t)k * n, (size_t)n * 8, hipMemcpyDeviceToDevice,
-                                 s.stream));
+    const double* st = s.stash.as<double>();
+    for (int slot = COSMO_POS; slot <= COSMO_ACCEL; ++slot)
+    {
+        const SlotDesc& d = *slotDesc(slot);
+        for (int c = 0; c < d.ncomp; ++c)
+            HIP_CHECK(hipMemcpyAsync((s.*d.comp[c]).as<double>(), st + (size_t)(d.firstInput + c) * n, (size_t)n * 8,
+                                     hipMemcpyDeviceToDevice, s.stream));
+    }
     HIP_CHECK(hipStreamSynchronize(s.stream));
+    resetInputTags(s);
 }
 
 // =========================================================================================

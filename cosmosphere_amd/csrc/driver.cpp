@@ -1,6 +1,7 @@
 /*
- * driver.cpp -- standalone driver of the density solve, linked against libcosmoSPHere.a the
- * way Phantom links it (C ABI only).  Command line and report follow the reference's
+ * driver.cpp -- standalone driver of the density solve, linked against libcosmoSPHere.a (the C
+ * slot transfers and the C++ solveDensH of include/cosmosphere.hpp, as the reference's driver
+ * calls them, src/main.cu:117-141).  Command line and report follow the reference's
  * build/density_hip (src/main.cu:143-171):
  *
  *   density_hip lattice <npartx> [output_h_file | no_output] [jiggle]
@@ -17,7 +18,7 @@
  * Extensions: "mixed" selects the mixed precision form; "--check-read <datafile>" only reads
  * the file and prints its count, particle mass and sums (no device needed).
  */
-#include "../../include/cosmosphere.h"
+#include "../../include/cosmosphere.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -165,7 +166,8 @@ int solve(const Particles& p, std::vector<double>& h, std::vector<double>& rho, 
     const std::vector<double> zero(3 * (size_t)n, 0.0);
     cosmo_upload(COSMO_VEL, zero.data(), n);
     cosmo_upload(COSMO_ACCEL, zero.data(), n);
-    const int iters = cosmo_ext_solve_dens_h(n, p.pmass, mode, 0, 0, nullptr, 1e-4, -1.0);
+    const DensTimings t = solveDensH(n, p.pmass, mode == 1 ? KernelMode::WARP_PER_LEAF : KernelMode::FLAT_PARTICLE);
+    const int iters = t.itersRun;
     cosmo_download(COSMO_HSML, h.data(), n);
     std::vector<double> dens(2 * (size_t)n);
     cosmo_download(COSMO_DENS_OUT, dens.data(), n);

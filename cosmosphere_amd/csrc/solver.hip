@@ -21,6 +21,7 @@
 #include "loop.hpp"
 #include "solver.hpp"
 #include "tree.hpp"
+#include "xfer.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -54,13 +55,16 @@ void sizeParticleArrays(State& s, int n)
     ensureStream(s);
     const size_t m = (size_t)std::max(n, 1) + 1;
     for (DevBuf* b : {&s.x, &s.y, &s.z, &s.h, &s.vx, &s.vy, &s.vz, &s.ax, &s.ay, &s.az, &s.rho, &s.gradh, &s.divv, &s.xi,
-                      &s.ddivvdt, &s.pro2, &s.cs, &s.alpha, &s.u, &s.fx, &s.fy, &s.fz, &s.f4, &s.vsig, &s.divvF, &s.stage,
+                      &s.ddivvdt, &s.pro2, &s.cs, &s.alpha, &s.u, &s.fx, &s.fy, &s.fz, &s.f4, &s.vsig, &s.divvF,
                       &s.hsqinv, &s.hinv, &s.rhoh, &s.rho1, &s.grkfac, &s.pres, &s.auterm, &s.divfac})
         b->get<double>(m);
     s.order.get<int>(m);
     s.particleLeaf.get<int>(m);
     s.active.get<uint8_t>(m);
     s.sizedFor = n;
+    // the arrays may have moved: no order and no resident inputs survive a resize
+    s.haveOrder = 0;
+    resetInputTags(s);
 }
 
 struct Ev
@@ -149,7 +153,11 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
     int* ovf = s.ovf.get<int>(4);
     fillU32(reinterpret_cast<uint32_t*>(ovf), 0u, 4, st);
     s.t = Timings{};
-    buildTree(s, withGradients, periodicBox, sortScratch());
+    // inputs uploaded in the resident order (cosmo_upload after a solve of the same set) are
+    // re-sorted incrementally; the rest are brought to one form first
+    const bool incremental = prepareInputs(s, withGradients);
+    buildTree(s, withGradients, periodicBox, sortScratch(), incremental);
+    resetInputTags(s);
     t = s.t;
     if (s.treeOnly) return t;
     const int n = s.n, na = s.totalAlive, L = s.nLeaves;

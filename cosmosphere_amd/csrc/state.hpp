@@ -52,6 +52,7 @@ struct Timings
     double kernLaunches = 0;                         // NR pair-kernel launches
     double closurePairs = 0, closureCand = 0;
     double graphReplays = 0, graphCaptures = 0;      // device-driven loop
+    double resortMovers = -1;                        // incremental re-sort: particles re-sorted from scratch (-1: full sort)
 };
 
 // One NR pass: active leaves, work items, particles iterating at its start, NR candidates and
@@ -74,6 +75,7 @@ struct ForceTimings
 };
 
 enum Precision : int { kPlainFP64 = 0, kMixed = 1 };
+enum InputTag : uint8_t { kTagCaller = 0, kTagAligned = 1 };
 enum Mapping : int { kMapAuto = 0, kMapWavePerLeaf = 1, kMapFlat = 2 };
 
 struct State
@@ -88,10 +90,21 @@ struct State
     DevBuf pro2, cs, alpha, u;
     DevBuf fx, fy, fz, f4, vsig, divvF;
     DevBuf order;       // sorted index -> caller index (int32)
+    DevBuf orderAlt;    // the next order while the incremental re-sort composes it
     int    haveOrder = 0;
-    DevBuf stage;       // transfer staging (n doubles)
+    int    orderSets = 0;   // sets of the solve that built `order`
+    DevBuf stage;       // transfer staging (up to 6 components of n doubles)
     DevBuf stash;       // pristine copy of POS|HSML|VEL|ACCEL for repeated benchmark solves
     int    stashN = 0;
+    // input fields x y z h vx vy vz ax ay az: kTagCaller (caller's order) or kTagAligned
+    // (the order `order` describes: entry i is caller particle order[i])
+    uint8_t inTag[10] = {};
+    int    resident = 1;     // cosmo_upload of input slots lands in the resident order when it can
+    int    xferThreads = 8;  // host threads of the page-locked transfer staging
+    int    ringNext = 0;     // next staging chunk
+    unsigned ringBusy = 0;   // staging chunks with a DMA possibly in flight
+    // incremental re-sort work arrays
+    DevBuf rsStay, rsPos, rsKey, rsCal, rsIdx, rsSortKey, rsSortVal;
 
     // ---- sets ----
     int nSets = 0;
@@ -182,7 +195,8 @@ struct State
     std::vector<PassRecord> passLog;   // one record per NR pass of the last solve
     int      closureOn = -1;   // -1: on for mixed, off for plain
     int      treeOnly = 0;     // verification: stop after the tree build
-    double   uploadSeconds = 0.0, downloadSeconds = 0.0;
+    double   uploadSeconds = 0.0, downloadSeconds = 0.0;   // host-timed transfers since the last stats line
+    double   uploadTotal = 0.0, downloadTotal = 0.0;       // ... and since the process started
     Timings      t;
     ForceTimings ft;
 

@@ -529,7 +529,7 @@ static double elapsed(hipEvent_t a, hipEvent_t b)
     return ms * 1e-3;
 }
 
-void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch& sc)
+void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch& sc, bool incremental)
 {
     hipStream_t st = s.stream;
     sc.err = s.ovf.as<int>() + 2;   // ovf[2]: sort / tree consistency errors
@@ -597,7 +597,19 @@ void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch
         LAUNCH_CHECK();
     }
     HIP_CHECK(hipEventRecord(ev.e[2], st));
-    if (radixSortPairs(keys, keysAlt, vals, valsAlt, n, 64, sc, st))
+    // Incremental re-sort (inputs in the previous solve's order): sorted keys into keysAlt,
+    // the input position of each sorted entry into valsAlt, the caller index into orderAlt.
+    s.t.resortMovers = -1;
+    if (incremental && n > 0)
+    {
+        int* orderNext = s.orderAlt.get<int>((size_t)n + 1);
+        s.t.resortMovers = resortIncremental(s, keys, keysAlt, valsAlt, orderNext, sc);
+        std::swap(s.keys, s.keysAlt);
+        std::swap(s.vals, s.valsAlt);
+        keys = s.keys.as<uint64_t>(); keysAlt = s.keysAlt.as<uint64_t>();
+        vals = s.vals.as<uint32_t>(); valsAlt = s.valsAlt.as<uint32_t>();
+    }
+    else if (radixSortPairs(keys, keysAlt, vals, valsAlt, n, 64, sc, st))
     {
         std::swap(s.keys, s.keysAlt);
         std::swap(s.vals, s.valsAlt);
@@ -657,8 +669,17 @@ void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch
     int* order = s.order.get<int>(n + 1);
     if (n > 0)
     {
-        valsToOrder<<<iceil(n, 256), 256, 0, st>>>(vals, order, n);
-        LAUNCH_CHECK();
+        // gather index: the caller index (full sort of caller-order inputs) or the input
+        // position (incremental re-sort of inputs in the previous order; the new order was
+        // composed by the re-sort)
+        const int* gidx = reinterpret_cast<const int*>(vals);
+        if (incremental) std::swap(s.order, s.orderAlt);
+        else
+        {
+            valsToOrder<<<iceil(n, 256), 256, 0, st>>>(vals, order, n);
+            LAUNCH_CHECK();
+            gidx = order;
+        }
         std::vector<DevBuf*> fields = {&s.x, &s.y, &s.z, &s.h};
         if (withVelAcc) for (DevBuf* f : {&s.vx, &s.vy, &s.vz, &s.ax, &s.ay, &s.az}) fields.push_back(f);
         // gather into the persistent alternates, then swap them in (passes of at most 10)
@@ -669,11 +690,12 @@ void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch
             a.src[k] = fields[k]->as<double>();
             a.dst[k] = s.permAlt[k].get<double>(std::max(n, s.sizedFor) + 1);
         }
-        permuteKernel<<<iceil(n, 256), 256, 0, st>>>(a, order, n);
+        permuteKernel<<<iceil(n, 256), 256, 0, st>>>(a, gidx, n);
         LAUNCH_CHECK();
         for (int k = 0; k < a.count; ++k) std::swap(*fields[k], s.permAlt[k]);
     }
     s.haveOrder = 1;
+    s.orderSets = B;
     HIP_CHECK(hipEventRecord(ev.e[4], st));
 
     // ---- internal nodes ----

@@ -14,7 +14,17 @@ struct PassCtl;
 // Box, keys, sort, permutation (x, y, z, h and, if withVelAcc, v and a), leaf tree, node
 // geometry, layout and particle -> leaf map for the sets described in s.
 // periodicBox: nullptr (open, fitted boxes) or 6 doubles per set.
-void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch& sc);
+// incremental: the inputs are in the previous solve's order (x, y, z, h and, with withVelAcc,
+// v and a), and the previous tree is still in s; the keys are then re-sorted incrementally
+// (resort.hip) with the same result as the full sort.
+void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch& sc, bool incremental = false);
+
+// Incremental re-sort of keys given in the previous solve's order (s.order, the previous
+// tree's leaves and particle -> leaf map): sorted keys, the input position of each sorted
+// entry and its caller index (the new order).  Returns the number of particles that left
+// their previous leaf's key range (sorted from scratch).
+int resortIncremental(State& s, const uint64_t* keys, uint64_t* keysOut, uint32_t* idxOut, int* orderOut,
+                      SortScratch& sc);
 
 // hmax of the given leaves (leaves == nullptr: all of them); with `state`, also the max h of
 // their particles in state != 0 into s.hRemLeaf (the reach of tight lists).
