@@ -31,7 +31,7 @@ ABI_SYMBOLS = ["densityiterate_gpu_c", "cosmo_kernel_radius", "force_gpu_c", "co
 EXT_SYMBOLS = ["cosmo_ext_solve_dens_h", "cosmo_ext_set_option", "cosmo_ext_get_stat",
                "cosmo_ext_get_array", "cosmo_ext_stash_inputs", "cosmo_ext_restore_inputs",
                "cosmo_ext_sync", "cosmo_batch_init", "cosmo_batch_solve", "cosmo_ext_autotune",
-               "cosmo_batch_autotune"]
+               "cosmo_batch_autotune", "cosmo_download_noalloc"]
 
 _ARRAY_DTYPES = {"keys": np.uint64, "order": np.int32, "leafStart": np.uint64, "leafLevel": np.int32,
                  "leafSet": np.int32, "layout": np.uint32, "particleLeaf": np.int32,
@@ -59,6 +59,8 @@ def _bind(L):
     L.cosmo_slot_ncomp.restype = ctypes.c_int
     for f in (L.cosmo_upload, L.cosmo_upload_sorted, L.cosmo_download):
         f.argtypes = [ctypes.c_int, c_dp, ctypes.c_int]
+    L.cosmo_download_noalloc.argtypes = [ctypes.c_int, c_dp, ctypes.c_int]
+    L.cosmo_download_noalloc.restype = ctypes.c_int
     L.cosmo_pin_host.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
     L.cosmo_unpin_host.argtypes = [ctypes.c_void_p]
     L.cosmo_ext_solve_dens_h.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -126,6 +128,14 @@ class Library:
     def download(self, slot, n):
         out = np.empty(NCOMP[slot] * int(n))
         self.L.cosmo_download(int(slot), _dptr(out), int(n))
+        return out.reshape(NCOMP[slot], int(n))
+
+    def download_noalloc(self, slot, n):
+        """cosmo_download_noalloc: as download, through the library's static device buffer."""
+        out = np.empty(NCOMP[slot] * int(n))
+        rc = self.L.cosmo_download_noalloc(int(slot), _dptr(out), int(n))
+        if rc != 0:
+            raise ValueError("cosmo_download_noalloc refused n=%d" % n)
         return out.reshape(NCOMP[slot], int(n))
 
     def densityiterate(self, n, pmass, periodic_box=None, tolh=1e-4, hfact=-1.0):

@@ -230,6 +230,29 @@ extern "C" void cosmo_download(int slot, double* host, int n)
     s.downloadTotal += secondsSince(t0);
 }
 
+// cosmo_download allocating nothing (no counterpart in the reference): for callers that cannot
+// allocate.  Sizes beyond the static buffer are refused before any device work.
+extern "C" int cosmo_download_noalloc(int slot, double* host, int n)
+{
+    if (n > kNoAllocMaxParticles)
+    {
+        std::fprintf(stderr,
+                     "cosmo_download_noalloc: n=%d exceeds the static scatter buffer of %d particles; nothing "
+                     "copied (cosmo_download has no such limit)\n",
+                     n, kNoAllocMaxParticles);
+        return -1;
+    }
+    if (n <= 0) return 0;
+    State& s = state();
+    const SlotDesc& d = slotFor(s, slot, n, "cosmo_download_noalloc", true);
+    // an ordering exists, so a solve has run and created the stream
+    const auto t0 = Clock::now();
+    transferOutNoAlloc(s, d, host, n);
+    s.downloadSeconds += secondsSince(t0);
+    s.downloadTotal += secondsSince(t0);
+    return 0;
+}
+
 // Page-locking is best effort in the reference (pin_c_api.cu:22-34: failure is cleared and
 // ignored).  Registering memory the caller already owns is refused on this platform, and a
 // refused registration leaves an error for the next runtime call to report; the copies go

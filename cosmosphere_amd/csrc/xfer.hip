@@ -306,6 +306,58 @@ void transferOut(State& s, const SlotDesc& d, double* host, int n)
 }
 
 // ---------------------------------------------------------------------------------------
+// Allocation-free download: each component is scattered into a static device buffer (the
+// array `scratch`, one copy per GPU, held until the process ends) and copied from there
+// straight into the caller's memory; no staging buffer, page-locked ring or thread pool.
+// The scatter writes every entry of [0, n) before the copy reads it, so nothing needs
+// clearing between calls.  Calls on one GPU share the buffer: overlapping calls race.
+// ---------------------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kNoAllocMaxParticles];
+
+namespace
+{
+constexpr int kMaxDevices = 64;
+double* gScratch[kMaxDevices] = {};   // device address of `scratch` on each GPU
+
+// The address of `scratch` on the current device, looked up on the first call there (the
+// array's host-side name is not a device address and must never reach a kernel).
+double* scratchOnDevice()
+{
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+    {
+        std::fprintf(stderr, "FATAL: cosmo_download_noalloc: device %d beyond the %d this library tracks\n", dev,
+                     kMaxDevices);
+        std::abort();
+    }
+    if (!gScratch[dev])
+    {
+        void* p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        gScratch[dev] = static_cast<double*>(p);
+    }
+    return gScratch[dev];
+}
+} // namespace
+
+void transferOutNoAlloc(State& s, const SlotDesc& d, double* host, int n)
+{
+    double* buf = scratchOnDevice();
+    for (int c = 0; c < d.ncomp; ++c)
+    {
+        CompPtrs src{};
+        src.p[0] = (s.*d.comp[c]).as<double>();
+        scatterSlotKernel<<<iceil(n, 256), 256, 0, s.stream>>>(src, s.order.as<int>(), buf, 1, n);
+        LAUNCH_CHECK();
+        // stream order: the next component's scatter waits for this copy
+        HIP_CHECK(hipMemcpyAsync(host + (size_t)c * n, buf, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                                 s.stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+}
+
+// ---------------------------------------------------------------------------------------
 // Resident order
 // ---------------------------------------------------------------------------------------
 bool residentEligible(const State& s, int n)

@@ -43,6 +43,11 @@ void transferIn(State& s, const SlotDesc& d, const double* host, int n, bool ali
 // Device -> host block, scattered through `order` into the caller's order.
 void transferOut(State& s, const SlotDesc& d, double* host, int n);
 
+// Device -> host block like transferOut, allocating nothing: the scatter goes through a static
+// device buffer of kNoAllocMaxParticles doubles per GPU (n must not exceed it).
+constexpr int kNoAllocMaxParticles = 4194304;
+void transferOutNoAlloc(State& s, const SlotDesc& d, double* host, int n);
+
 // True when cosmo_upload of an input slot should land aligned (resident order on, an order
 // for exactly these n particles from a single-set solve).
 bool residentEligible(const State& s, int n);

@@ -84,6 +84,16 @@ void cosmo_unpin_host(void* ptr);
 
 /* ---- extensions (no counterpart in the reference's C ABI) ------------------------------- */
 
+/* cosmo_download that allocates nothing.  No counterpart in the reference: for callers that
+ * cannot allocate (HIP graph capture, serving loops, harnesses that forbid allocation).  Each
+ * component is scattered into a static device buffer of COSMO_NOALLOC_MAX_PARTICLES doubles
+ * (32 MiB, one per GPU, held until the process ends) and copied from it into host.  Returns 0,
+ * or -1 with a message on stderr and nothing copied when n exceeds COSMO_NOALLOC_MAX_PARTICLES;
+ * misuse aborts as cosmo_download.  Calls on one GPU share the buffer: calls that overlap
+ * (several host threads) race. */
+#define COSMO_NOALLOC_MAX_PARTICLES 4194304
+int cosmo_download_noalloc(int slot, double* host, int n);
+
 /* The reference's C++ solveDensH(n, pmass, mode, withGradients, periodicBox, tolh, hfact)
  * (include/density.hpp:85-91) with C linkage: mode 0 = FLAT_PARTICLE, 1 = WARP_PER_LEAF.
  * Returns the Newton iterations run. */
