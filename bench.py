@@ -154,17 +154,22 @@ def fp64_closure_reference(lib, c):
     return lib.array("h"), lib.array("rho")
 
 
-def cpu_baseline(c, gpu_h_fp64, gpu_h_mixed, seconds):
-    """FP64 oracle on a bounded sample: all particles of randomly chosen leaves, solved over
-    the full tree (in FLAT mode a particle's h depends on positions only, so the sample's h
-    equal the full solve's).  Rate = sampled particles / (sample NR time + the tree build's
-    share for the sample)."""
+def build_oracle(c):
+    """The FP64 oracle's tree of the workload (keys, sort, leaf tree), timed."""
     from oracle import oracle as O
     o = O.Oracle(quintic=c["quintic"])
     o.set_particles(c["x"], c["y"], c["z"], c["h"])
     t = time.perf_counter()
     o.build()
-    t_tree = time.perf_counter() - t
+    return o, time.perf_counter() - t
+
+
+def cpu_baseline(c, o, t_tree, gpu_h_fp64, gpu_h_mixed, seconds):
+    """FP64 oracle on a bounded sample: all particles of randomly chosen leaves, solved over
+    the full tree (in FLAT mode a particle's h depends on positions only, so the sample's h
+    equal the full solve's).  Rate = sampled particles / (sample NR time + the tree build's
+    share for the sample)."""
+    from oracle import oracle as O
     na = o.geti("nAlive")
     pl = o.get("particleLeaf")[:na]
     rng = np.random.default_rng(0)
@@ -194,12 +199,60 @@ def cpu_baseline(c, gpu_h_fp64, gpu_h_mixed, seconds):
            "p99_rel_err_h_mixed": float(np.percentile(np.abs(gpu_h_mixed[idx] - h_ref) / h_ref, 99))
            if gpu_h_mixed is not None else None,
            "particles": int(npart)}
+    err["sample_idx"] = idx
+    err["sample_h"] = h_ref
+    err["sample_rho"] = o.get("rho")[idx]
     base = {"value": rate, "unit": "particle h-solves/s", "cores": O.num_threads(), "kind": "port",
             "sample": "FP64 oracle (oracle/sph_oracle.cpp, OpenMP) on all %d particles of %d randomly chosen leaves "
                       "(%.3f%% of the live particles), solved to convergence over the full %d-particle tree; tree "
                       "build %.2f s charged pro rata, NR %.2f s" % (npart, leaves.size, 100.0 * npart / na, na, t_tree,
                                                                      t_nr)}
     return base, err
+
+
+def parity_full(lib, c, o, sample):
+    """Full-size parity against the FP64 oracle (the GPU's plain FP64 path, the oracle's tree):
+    keys, order, leaf tree and layout bit for bit; the all-leaf gather lists at the GPU's
+    converged h (one FP64 solve with the gradient sweep, which rebuilds them) bit for bit; h and
+    rho of the sampled leaves' particles (the cpu_baseline sample), with the particles whose
+    Newton iteration stopped one step apart counted separately (|dh/h| above 1e-12: summation
+    order alone moves h by rounding, about 1e-15; a |dh/h| < tolh decision that rounding tipped
+    moves it by one Newton step, and rho, taken at the h before the last update, further)."""
+    t0 = time.perf_counter()
+    lib.set_option("precision", 0)
+    lib.set_option("closure", -1)
+    lib.restore_inputs()
+    lib.solve_dens_h(c["n"], c["pmass"], 0, True, None, 1e-4, c["hfact"])
+    out = {}
+    cmp = {"keys": "keys", "order": "order", "leafLevel": "leafLevel", "layout": "layout"}
+    for g, r in cmp.items():
+        a, b = lib.array(g), o.get(r)
+        out[g] = bool(a.size == b.size and np.array_equal(a, b))
+    ls, cs = lib.array("leafStart"), o.get("csTree")[:-1]
+    out["leafStart"] = bool(ls.size == cs.size and np.array_equal(ls, cs))
+    h_gpu = lib.array("h")
+    o.set_h_sorted(h_gpu)
+    o.hmax_all()
+    o.build_lists(symmetric=False)
+    jo, jl = lib.array("jOffset"), lib.array("jList")
+    ro, rl = o.get("jOffset"), o.get("jList")
+    out["gather_lists"] = bool(jo.size == ro.size and np.array_equal(jo.astype(np.int64), ro.astype(np.int64))
+                               and np.array_equal(jl, rl))
+    out["list_entries"] = int(rl.size)
+    out["leaves"] = int(o.geti("nLeaves"))
+    idx, h_ref, rho_ref = sample["sample_idx"], sample["sample_h"], sample["sample_rho"]
+    h_fp64, rho_fp64 = sample["gpu_h_fp64_sorted"], sample["gpu_rho_fp64_sorted"]
+    eh = np.abs(h_fp64[idx] - h_ref) / h_ref
+    flip = eh > 1e-12
+    er = np.abs(rho_fp64[idx] - rho_ref) / rho_ref
+    out["sample_particles"] = int(idx.size)
+    out["sample_flips"] = int(flip.sum())
+    out["sample_max_rel_err_h_unflipped"] = float(eh[~flip].max()) if (~flip).any() else 0.0
+    out["sample_max_rel_err_rho_unflipped"] = float(er[~flip].max()) if (~flip).any() else 0.0
+    out["sample_max_rel_err_h"] = float(eh.max())
+    out["seconds"] = time.perf_counter() - t0
+    out["bit_exact"] = all(out[k] for k in ("keys", "order", "leafLevel", "layout", "leafStart", "gather_lists"))
+    return out
 
 
 def bench_ensemble(args):
@@ -281,15 +334,115 @@ def bench_ensemble(args):
         "roofline": {"bound": "valu", "achieved": m["kernel_tflops"], "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": m["kernel_tflops"] / FP32_PEAK, "traffic": None, "kernel_avg_ms": m["kernel_avg_ms"]},
     }
+    if not args.no_cpu:
+        log("cpu baseline (oracle on a sample of the sets)")
+        res["cpu_baseline"] = ensemble_cpu_baseline(sets, args.cpu_seconds)
     print(json.dumps(res), flush=True)
+
+
+def ensemble_cpu_baseline(sets, seconds):
+    """FP64 oracle on whole sets of the ensemble, drawn at random until about `seconds` of work:
+    tree build and FLAT Newton-Raphson solve of each, one after another."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(0)
+    done, parts, t_all = [], 0, 0.0
+    for b in rng.permutation(len(sets)):
+        s = sets[b]
+        o = O.Oracle(quintic=False)
+        o.set_particles(s["x"], s["y"], s["z"], s["h"])
+        t = time.perf_counter()
+        o.build()
+        o.density_solve(s["pmass"], 1e-4, 1.2, 100, 0, False)
+        t_all += time.perf_counter() - t
+        parts += s["n"]
+        done.append(int(b))
+        if t_all > seconds:
+            break
+    return {"value": parts / t_all, "unit": "particle h-solves/s", "cores": O.num_threads(), "kind": "port",
+            "sample": "FP64 oracle (oracle/sph_oracle.cpp, OpenMP): %d whole sets (%d particles) solved one after "
+                      "another in %.2f s" % (len(done), parts, t_all)}
+
+
+def force_traffic(form):
+    """HBM bytes per launch of the force kernel on C5, from the committed rocprofv3 summary, or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "force_c5_%s_counters.md" % form)
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", open(path).read())
+    return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
+
+
+def force_cpu_baseline(c, st, seconds):
+    """FP64 oracle on a bounded sample of one carried C5 step: all particles of randomly chosen
+    leaves.  The density solve of the sample starts from the step's carried h (as the GPU's);
+    the force pass of the sample runs at the GPU FP64 path's converged h and d(rho)/d(h) (the
+    symmetric sums read h_j of neighbours outside the sample).  Rate = sampled particles / (sample
+    density time + sample force time + the tree build's share for the sample).  The same sample
+    checks the GPU FP64 step against the oracle at full size: h, and the forces and du/dt."""
+    from oracle import oracle as O
+    n, pmass, hfact = c["n"], c["pmass"], c["hfact"]
+    o = O.Oracle(quintic=True)
+    o.set_particles(st["x"], st["y"], st["z"], st["h_in"], v=st["v"], a=st["a_in"])
+    t = time.perf_counter()
+    o.build()
+    t_tree = time.perf_counter() - t
+    na = o.geti("nAlive")
+    pl = o.get("particleLeaf")[:na]
+    order = o.get("order")
+    rng = np.random.default_rng(0)
+
+    def run(frac):
+        leaves = np.unique(pl[rng.choice(na, size=max(1, int(frac * na)), replace=False)])
+        t0 = time.perf_counter()
+        o.density_solve_leaves(pmass, leaves, 1e-4, hfact, 100)
+        t1 = time.perf_counter()
+        return leaves, t1 - t0
+
+    frac = 2e-4
+    leaves, t_d = run(frac)
+    if t_d < 0.1 * seconds:
+        frac = min(1.0, frac * 0.3 * seconds / max(t_d, 1e-3))
+        leaves, t_d = run(frac)
+    lay = o.get("layout")
+    idx = np.concatenate([np.arange(lay[l], lay[l + 1]) for l in leaves])
+    h_cpu = o.get("h")[idx].copy()
+    # the force pass of the sample at the GPU's converged state
+    o.set_h_sorted(st["h_out"][order])
+    o.set_gradh_sorted(st["gradh_out"][order])
+    o.set_thermo(*st["thermo"])
+    t0 = time.perf_counter()
+    o.force_leaves(pmass, leaves, hfact, c["beta"], c["alphau"], 0, 1, 1)
+    t_f = time.perf_counter() - t0
+    npart = idx.size
+    rate = npart / (t_d + t_f + t_tree * npart / na)
+    cal = order[idx]
+    eh = np.abs(st["h_out"][cal] - h_cpu) / h_cpu
+    flip = eh > 1e-12
+    f_gpu = st["f_out"]
+    fscale = np.abs(f_gpu[:3]).max()
+    err = {"particles": int(npart), "h_flips": int(flip.sum()),
+           "max_rel_err_h_unflipped": float(eh[~flip].max()) if (~flip).any() else 0.0}
+    for k, nm in enumerate(("fx", "fy", "fz", "du")):
+        ref = o.get(nm)[idx]
+        scale = fscale if k < 3 else np.abs(f_gpu[3]).max()
+        err[nm + "_max_err_rel_to_max"] = float(np.max(np.abs(f_gpu[k][cal] - ref)) / scale)
+    base = {"value": rate, "unit": "particle density+force updates/s", "cores": O.num_threads(), "kind": "port",
+            "sample": "FP64 oracle (oracle/sph_oracle.cpp, OpenMP) on all %d particles of %d randomly chosen leaves "
+                      "(%.3f%% of the live particles) of the first carried step: density solve from the carried h "
+                      "%.2f s, force pass %.2f s, tree build %.2f s charged pro rata"
+                      % (npart, leaves.size, 100.0 * npart / na, t_d, t_f, t_tree)}
+    return base, err
 
 
 def bench_force(args, n_override=None):
     """C5: density + force on the 16M-particle Plummer mixture with velocities, through the
     reference's call sequence (densityiterate_gpu_c with gradients -> THERMO -> force_gpu_c), and
     a loop of 10 drift steps x += v dt that carries h and the accelerations forward (11 solves).
-    Mixed vs plain FP64; per-output errors of the mixed form against the plain FP64 GPU path
-    (itself pinned to the FP64 oracle by the GPU parity tests) at steps 1 and 10."""
+    Mixed vs plain FP64; per-output errors of the mixed form against the plain FP64 GPU path at
+    steps 1 and 10; the FP64 path's step 1 against the FP64 oracle on a sample (cpu_baseline).
+    Inputs are uploaded every step into the resident order (incremental re-sort); the library's
+    own transfer times (host-timed, PCIe included) are reported beside the resident step time."""
     from cosmosphere_amd.api import DENS_OUT, FORCE_OUT, THERMO
     t = time.perf_counter()
     c = workloads.config5() if n_override is None else workloads.config5(n=n_override)
@@ -301,7 +454,10 @@ def bench_force(args, n_override=None):
     lib = Library(c["kernel"])
     lib.arrays_init(n)
     keep_steps = (1, nsteps)
-    res, snaps = {}, {}
+    res, snaps, step1 = {}, {}, None
+    pos = np.empty((3, n))
+    th = np.empty((4, n))
+    th[1], th[2], th[3] = c["cs"], c["alpha"], c["u"]
     for name, prec in (("fp64", 0), ("mixed", 1)):
         if args.only not in ("both", name):
             continue
@@ -311,28 +467,27 @@ def bench_force(args, n_override=None):
         lib.set_option("tree_only", 0)
         for k, val in args.opts:
             lib.set_option(k, val)
-        x, y, z = c["x"].copy(), c["y"].copy(), c["z"].copy()
+        pos[0], pos[1], pos[2] = c["x"], c["y"], c["z"]
         h = c["h"].copy()
         a = np.zeros((3, n))
         rec = []
         snaps[name] = {}
         for step in range(nsteps + 1):
+            x0 = lib.stat("upload_s") + lib.stat("download_s")
             t0 = time.perf_counter()
-            lib.upload(POS, np.stack([x, y, z]))
+            lib.upload(POS, pos)
             lib.upload(HSML, h)
             lib.upload(VEL, v)
             lib.upload(ACCEL, a)
-            lib.sync()
             t1 = time.perf_counter()
             lib.densityiterate(n, pmass, tolh=1e-4, hfact=hfact)
             t2 = time.perf_counter()
             dens = {k: lib.stat(k) for k in ("total", "iters", "nrkern", "gradkern", "gjbuild", "jbuild", "pairs",
-                                             "candidates")}
+                                             "candidates", "resort_movers", "sort")}
             hnew = lib.download(HSML, n)[0]
-            rhoh = pmass * (hfact / hnew) ** 3
-            th = np.stack([(gamma - 1.0) * c["u"] / rhoh, c["cs"], c["alpha"], c["u"]])
+            np.multiply(np.power(hfact / hnew, 3), pmass, out=th[0])
+            np.divide((gamma - 1.0) * c["u"], th[0], out=th[0])
             lib.upload_sorted(THERMO, th)
-            lib.sync()
             t3 = time.perf_counter()
             lib.force(n, pmass, beta=c["beta"], alphau=c["alphau"], disc_viscosity=0, pdv_heating=1, shock_heating=1)
             t4 = time.perf_counter()
@@ -340,25 +495,40 @@ def bench_force(args, n_override=None):
                                             "force_candidates")}
             f = lib.download(FORCE_OUT, n)
             t5 = time.perf_counter()
-            rec.append({"step": step, "density_s": t2 - t1, "force_s": t4 - t3, "step_wall_s": t5 - t0, **dens, **frc})
-            log("  %s step %d: density %.1f ms (%d iters), force %.1f ms (kernel %.1f ms), step with transfers "
-                "%.1f ms" % (name, step, 1e3 * (t2 - t1), dens["iters"], 1e3 * (t4 - t3), 1e3 * frc["force_kernel"],
-                             1e3 * (t5 - t0)))
+            xfer = lib.stat("upload_s") + lib.stat("download_s") - x0
+            rec.append({"step": step, "density_s": t2 - t1, "force_s": t4 - t3, "step_wall_s": t5 - t0,
+                        "transfer_s": xfer, **dens, **frc})
+            log("  %s step %d: density %.1f ms (%d iters, re-sort movers %d), force %.1f ms (kernel %.1f ms), "
+                "transfers %.1f ms, step with transfers %.1f ms"
+                % (name, step, 1e3 * (t2 - t1), dens["iters"], dens["resort_movers"], 1e3 * (t4 - t3),
+                   1e3 * frc["force_kernel"], 1e3 * xfer, 1e3 * (t5 - t0)))
             if step in keep_steps:
                 d = lib.download(DENS_OUT, n)
                 snaps[name][step] = {"h": hnew.copy(), "rho": d[0].copy(), "f": f.copy()}
+                if name == "fp64" and step == 1 and not args.no_cpu:
+                    step1 = {"x": pos[0].copy(), "y": pos[1].copy(), "z": pos[2].copy(), "h_in": h.copy(),
+                             "v": v, "a_in": a.copy(), "h_out": hnew.copy(), "gradh_out": d[1].copy(),
+                             "f_out": f.copy(), "thermo": [th[0].copy(), th[1].copy(), th[2].copy(), th[3].copy()]}
             # drift, carrying h and the accelerations into the next solve
-            x, y, z = x + v[0] * dt, y + v[1] * dt, z + v[2] * dt
+            pos[0] += v[0] * dt
+            pos[1] += v[1] * dt
+            pos[2] += v[2] * dt
             h = hnew
             a = f[:3].copy()
         tot_d = sum(r["density_s"] for r in rec)
         tot_f = sum(r["force_s"] for r in rec)
         tot_k = sum(r["force_kernel"] for r in rec)
         fl = FORCE_FLOPS_CAND * sum(r["force_candidates"] for r in rec) + FORCE_FLOPS_ACC * sum(r["force_pairs"] for r in rec)
+        carried = rec[1:] if len(rec) > 1 else rec
         res[name] = {
             "particle_updates_per_s": n * len(rec) / (tot_d + tot_f),
             "ms_per_step": 1e3 * (tot_d + tot_f) / len(rec),
             "ms_per_step_with_transfers": 1e3 * sum(r["step_wall_s"] for r in rec) / len(rec),
+            "transfer_ms_per_step": 1e3 * sum(r["transfer_s"] for r in rec) / len(rec),
+            "ms_per_carried_step": 1e3 * sum(r["density_s"] + r["force_s"] for r in carried) / len(carried),
+            "ms_per_carried_step_with_transfers": 1e3 * sum(r["step_wall_s"] for r in carried) / len(carried),
+            "resort_movers_carried": [int(r["resort_movers"]) for r in rec[1:]],
+            "sort_ms_carried": 1e3 * sum(r["sort"] for r in carried) / len(carried),
             "density_ms_step0": 1e3 * rec[0]["density_s"],
             "density_ms_carried": 1e3 * sum(r["density_s"] for r in rec[1:]) / max(len(rec) - 1, 1),
             "iters_step0": rec[0]["iters"],
@@ -376,11 +546,21 @@ def bench_force(args, n_override=None):
         }
     out = {"metric": METRIC + " (density + force)", "unit": "particle density+force updates/s", "n_gpus": 1,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+           "steps": nsteps + 1, "warmup": 0,
            "config": {"workload": c["name"], "n": n, "kernel": c["kernel"], "solves": nsteps + 1,
                       "dt": dt, "gamma": gamma, "alpha_AV": 1.0, "beta": c["beta"], "parallelism": "single GPU"}}
     head = "mixed" if "mixed" in res else "fp64"
+    peak = FP32_PEAK if head == "mixed" else FP64_PEAK
+    traffic, traffic_src = force_traffic(head)
     out.update({"value": res[head]["particle_updates_per_s"], "ms_per_step": res[head]["ms_per_step"],
-                "dtype": "fp32 pair sums + fp64 Newton" if head == "mixed" else "fp64", head: res[head]})
+                "dtype": "fp32 pair sums + fp64 Newton" if head == "mixed" else "fp64", head: res[head],
+                "pair_interactions_per_s": res[head]["force_pair_interactions_per_s"],
+                "roofline": {"bound": "valu", "achieved": res[head]["force_kernel_tflops"], "peak": peak,
+                             "unit": "TFLOP/s", "frac": res[head]["force_kernel_tflops"] / peak, "traffic": traffic,
+                             "traffic_source": traffic_src,
+                             "kernel": "forceKernel%s (force pair sums)" % ("Mixed" if head == "mixed" else ""),
+                             "kernel_avg_ms": res[head]["force_kernel_ms"],
+                             "flops_convention": "12 per candidate + 120 per accepted pair"}})
     for k in res:
         out[k] = res[k]
     if "fp64" in res and "mixed" in res:
@@ -396,6 +576,11 @@ def bench_force(args, n_override=None):
                 e[nm + "_rel_to_max"] = {"max": float(d.max()), "p99": float(np.percentile(d, 99))}
             err["step%d" % step] = e
         out["mixed_vs_fp64_error"] = err
+    if step1 is not None:
+        log("cpu baseline (oracle sample of the first carried step)")
+        base, perr = force_cpu_baseline(c, step1, args.cpu_seconds)
+        out["cpu_baseline"] = base
+        out["parity_sample_fp64_step1"] = perr
     print(json.dumps(out), flush=True)
 
 
@@ -412,9 +597,10 @@ def main():
     ap.add_argument("--tune-budget", type=float, default=None,
                     help="seconds of autotuning per precision form before the timed steps (default: 60 for C4, "
                          "45 for C1-C3, 0 for C5)")
-    ap.add_argument("--tune-forms", default="mixed",
-                    help="comma-separated precision forms of a single-set workload to autotune (default: mixed; the "
-                         "FP64 path keeps the library's default options)")
+    ap.add_argument("--tune-forms", default="mixed,fp64",
+                    help="comma-separated precision forms of a single-set workload to autotune (default: both, so "
+                         "mixed_vs_fp64 compares tuned paths)")
+    ap.add_argument("--no-parity", action="store_true", help="skip the full-size parity leg of C1-C3")
     ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
                     help="library option (cosmo_ext_set_option), e.g. loop=0 for the host-driven NR loop")
     args = ap.parse_args()
@@ -526,10 +712,17 @@ def main():
                                       "particles": na}
     if not args.no_cpu:
         log("cpu baseline (oracle sample)")
-        base, err = cpu_baseline(c, res.get("fp64", {}).get("h_sorted"), res.get("mixed", {}).get("h_sorted"),
-                                 args.cpu_seconds)
+        o, t_tree = build_oracle(c)
+        base, err = cpu_baseline(c, o, t_tree, res.get("fp64", {}).get("h_sorted"),
+                                 res.get("mixed", {}).get("h_sorted"), args.cpu_seconds)
         out["cpu_baseline"] = base
-        out["parity_sample"] = err
+        if "fp64" in res and not args.no_parity:
+            log("full-size parity (keys, order, tree, gather lists; h and rho of the sample)")
+            err["gpu_h_fp64_sorted"] = res["fp64"]["h_sorted"]
+            err["gpu_rho_fp64_sorted"] = res["fp64"]["rho_sorted"]
+            out["parity_full"] = parity_full(lib, c, o, err)
+            log("  %s" % json.dumps(out["parity_full"]))
+        out["parity_sample"] = {k: v for k, v in err.items() if not isinstance(v, np.ndarray)}
     print(json.dumps(out), flush=True)
 
 

@@ -48,6 +48,7 @@ def lib():
         L.orc_set_acc.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.orc_set_thermo_sorted.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_set_h_sorted.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_set_gradh_sorted.argtypes = [ctypes.c_void_p, _dp]
         L.orc_build.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.orc_get_int.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_get_int.restype = ctypes.c_int
@@ -67,6 +68,8 @@ def lib():
         L.orc_closure.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_force.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_force_leaves.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int]
         L.orc_brute_density.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                         ctypes.c_double, _dp, _dp]
         _LIB = L
@@ -169,6 +172,9 @@ class Oracle:
     def set_h_sorted(self, h):
         lib().orc_set_h_sorted(self._p, _c(h))
 
+    def set_gradh_sorted(self, gradh):
+        lib().orc_set_gradh_sorted(self._p, _c(gradh))
+
     def build(self, periodic_box=None):
         box = np.zeros(6) if periodic_box is None else _c(periodic_box)
         lib().orc_build(self._p, int(periodic_box is not None), box)
@@ -208,6 +214,12 @@ class Oracle:
 
     def closure(self, pmass):
         lib().orc_closure(self._p, float(pmass))
+
+    def force_leaves(self, pmass, leaves, hfact=-1.0, beta=2.0, alphau=1.0, disc=0, pdv=1, shock=1):
+        """Force pass for the particles of the given leaves only (h and gradh of all set)."""
+        lv = np.ascontiguousarray(leaves, dtype=np.int32)
+        lib().orc_force_leaves(self._p, float(pmass), float(hfact), float(beta), float(alphau), int(disc), int(pdv),
+                               int(shock), lv, lv.size)
 
     def force(self, pmass, hfact=-1.0, beta=2.0, alphau=1.0, disc=0, pdv=1, shock=1):
         lib().orc_force(self._p, float(pmass), float(hfact), float(beta), float(alphau),

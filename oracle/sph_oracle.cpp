@@ -498,20 +498,28 @@ static void walk(const State& s, bool sym, int iLeaf, std::vector<int>& out)
     } while (node != -1);
 }
 
-// CSR j-lists over all leaves: include/tree.cuh:362-389 (symmetric: src/force.cu:19-88)
-void buildLists(State& s, bool sym)
+// CSR j-lists over all leaves: include/tree.cuh:362-389 (symmetric: src/force.cu:19-88).
+// only (bench sample): the lists of the other leaves are left empty.
+void buildLists(State& s, bool sym, const std::vector<int>* only = nullptr)
 {
     const int L = nLeaves(s);
     if (sym) upsweep(s, 0);
     std::vector<std::vector<int>> lists(L);
+    std::vector<int> which;
+    if (only) which = *only;
+    else
+    {
+        which.resize(L);
+        std::iota(which.begin(), which.end(), 0);
+    }
 #pragma omp parallel
     {
         std::vector<int> tmp;
 #pragma omp for schedule(dynamic, 16)
-        for (int l = 0; l < L; ++l)
+        for (size_t a = 0; a < which.size(); ++a)
         {
-            walk(s, sym, l, tmp);
-            lists[l] = tmp;
+            walk(s, sym, which[a], tmp);
+            lists[which[a]] = tmp;
         }
     }
     s.jOffset.assign(L + 1, 0);
@@ -822,8 +830,9 @@ int densitySolve(State& s, double pmass, double tolh, double hfact, int maxIter,
 // ---------------------------------------------------------------------------------------
 // Force pass: src/force.cu:92-137 (prep) and :145-383 (pair sums)
 // ---------------------------------------------------------------------------------------
+// only (bench sample): the particles of these leaves; the others' outputs stay zero.
 void force(State& s, double pmass, double hfact, double beta, double alphau, int disc, int pdv,
-           int shock)
+           int shock, const std::vector<int>* only = nullptr)
 {
     const int n = s.n;
     if (!(hfact > 0.0)) hfact = s.k.hfactDefault;
@@ -853,9 +862,15 @@ void force(State& s, double pmass, double hfact, double beta, double alphau, int
     s.fx.assign(n, 0); s.fy.assign(n, 0); s.fz.assign(n, 0);
     s.f4.assign(n, 0); s.vsig.assign(n, 0); s.divvF.assign(n, 0);
     const bool full = pdv && shock;
+    std::vector<int> parts;
+    if (only)
+        for (int l : *only)
+            for (unsigned j = s.layout[l]; j < s.layout[l + 1]; ++j) parts.push_back((int)j);
+    const int np = only ? (int)parts.size() : n;
 #pragma omp parallel for schedule(dynamic, 64)
-    for (int i = 0; i < n; ++i)
+    for (int ip = 0; ip < np; ++ip)
     {
+        const int i = only ? parts[ip] : ip;
         const double hi = s.h[i];
         if (!(hi > 0.0)) continue;
         const double xi = s.x[i], yi = s.y[i], zi = s.z[i];
@@ -1074,6 +1089,13 @@ void orc_set_h_sorted(void* p, const double* h)
     s.h.assign(h, h + s.n);
 }
 
+// d(rho)/d(h) in Hilbert order (bench sample: the force pass at another solver's h)
+void orc_set_gradh_sorted(void* p, const double* gradh)
+{
+    State& s = *static_cast<State*>(p);
+    s.gradh.assign(gradh, gradh + s.n);
+}
+
 void orc_build(void* p, int periodic, const double* box6)
 {
     orc::build(*static_cast<State*>(p), periodic, box6);
@@ -1202,6 +1224,18 @@ void orc_force(void* p, double pmass, double hfact, double beta, double alphau, 
     State& s = *static_cast<State*>(p);
     orc::buildLists(s, true);
     orc::force(s, pmass, hfact, beta, alphau, disc, pdv, shock);
+}
+
+// Force pass for the particles of the given leaves only (bench sample; the lists of the other
+// leaves are not built and their outputs stay zero).  h and gradh of every particle must be set.
+void orc_force_leaves(void* p, double pmass, double hfact, double beta, double alphau, int disc, int pdv, int shock,
+                      const int* leaves, int nLeaves)
+{
+    State& s = *static_cast<State*>(p);
+    const std::vector<int> sub(leaves, leaves + nLeaves);
+    orc_hmax_all(p);
+    orc::buildLists(s, true, &sub);
+    orc::force(s, pmass, hfact, beta, alphau, disc, pdv, shock, &sub);
 }
 
 // Brute-force O(N^2) density sums at the given h (independent of tree and lists).
