@@ -208,7 +208,7 @@ class Library:
 
     # ---- autotuner ----
     TUNE_OPTIONS = ("loop", "bucket", "mapping", "sparse", "tail_leaves", "loop_blocks_scale", "graph_passes", "lpt",
-                    "cull", "units")
+                    "cull", "units", "tile_depth")
 
     def autotune(self, n, pmass, tolh=1e-4, hfact=-1.0, periodic_box=None, budget_s=60.0):
         """Pick the solve's speed options for the uploaded set; returns the options chosen."""

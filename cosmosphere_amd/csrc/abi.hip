@@ -308,6 +308,12 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "sweep_split") s.sweepSplit = std::max(0, (int)value);
     else if (nm == "list_expand_leaves") s.listExpandLeaves = (int)value;
     else if (nm == "resident") s.resident = (int)value;
+    else if (nm == "force_rounds") s.forceRounds = (int)value;
+    else if (nm == "tile_depth")
+    {
+        if (value != 64 && value != 128) fatal("cosmo_ext_set_option: tile_depth must be 64 or 128");
+        s.tileDepth = (int)value;
+    }
     else if (nm == "xfer_threads") s.xferThreads = std::max(1, (int)value);
     else
     {
@@ -374,6 +380,8 @@ extern "C" double cosmo_ext_get_stat(const char* name)
     if (nm == "sweep_split") return s.sweepSplit;
     if (nm == "sweep_items") return s.sweepHeavyItems;
     if (nm == "resident") return s.resident;
+    if (nm == "force_rounds") return s.forceRounds;
+    if (nm == "tile_depth") return s.tileDepth;
     if (nm == "resort_movers") return t.resortMovers;
     if (nm == "upload_s") return s.uploadTotal;
     if (nm == "download_s") return s.downloadTotal;

@@ -13,6 +13,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <limits>
+#include <type_traits>
 
 // Abort on any runtime error, naming the call site (reference convention,
 // include/gpu_check.hpp:14-22).
@@ -171,35 +173,115 @@ __device__ inline uint64_t ballot64(bool p) { return __ballot(p); }
 
 __device__ inline uint64_t lanemaskLt() { return (uint64_t(1) << laneId()) - 1; }
 
+// Cross-lane moves by DPP (data-parallel primitives: the operand of a VALU instruction read
+// from another lane of the same row, no LDS traffic).  GFX9 controls: row_shr:n shifts within
+// rows of 16 lanes; row_bcast:15 / row_bcast:31 hand lane 15 of each row (lane 31) to the next
+// row (rows 2 and 3).  Lanes without a source, or outside row_mask, receive `old`.
+constexpr int kDppRowShr1    = 0x111;
+constexpr int kDppRowShr2    = 0x112;
+constexpr int kDppRowShr4    = 0x114;
+constexpr int kDppRowShr8    = 0x118;
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+
+template<int CTRL, int ROW_MASK, class T>
+__device__ inline T dppMove(T old, T v)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    if constexpr (sizeof(T) == 4)
+    {
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v),
+                                                                 CTRL, ROW_MASK, 0xF, false));
+    }
+    else
+    {
+        const long long o = __builtin_bit_cast(long long, old), x = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, ROW_MASK, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, ROW_MASK, 0xF, false);
+        return __builtin_bit_cast(T, ((long long)hi << 32) | (long long)(uint32_t)lo);
+    }
+}
+
+// Inclusive scan over the 64 lanes with an associative op and its identity: Hillis-Steele
+// within each row (shifts 1, 2, 4, 8), then the rows joined by the two broadcasts.
+template<class T, class Op>
+__device__ inline T waveScanDpp(T v, T id, Op op)
+{
+    v = op(v, dppMove<kDppRowShr1, 0xF>(id, v));
+    v = op(v, dppMove<kDppRowShr2, 0xF>(id, v));
+    v = op(v, dppMove<kDppRowShr4, 0xF>(id, v));
+    v = op(v, dppMove<kDppRowShr8, 0xF>(id, v));
+    v = op(v, dppMove<kDppRowBcast15, 0xA>(id, v));
+    v = op(v, dppMove<kDppRowBcast31, 0xC>(id, v));
+    return v;
+}
+
+// The value of lane 63, in every lane (a scalar read).
+template<class T>
+__device__ inline T lastLane(T v)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), kWave - 1));
+    else
+    {
+        const long long x = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_readlane((int)x, kWave - 1), hi = __builtin_amdgcn_readlane((int)(x >> 32), kWave - 1);
+        return __builtin_bit_cast(T, ((long long)hi << 32) | (long long)(uint32_t)lo);
+    }
+}
+
+// Sum over the wave, in every lane (the scan's total in lane 63).
 template<class T>
 __device__ inline T waveReduceSum(T v)
 {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    return lastLane(waveScanDpp(v, T(0), [](T a, T b) { return a + b; }));
+}
+
+// Identities of max and min.
+template<class T>
+__device__ inline T lowestOf()
+{
+    if constexpr (std::is_floating_point<T>::value) return T(-__builtin_inf());
+    else return std::numeric_limits<T>::lowest();
+}
+
+template<class T>
+__device__ inline T highestOf()
+{
+    if constexpr (std::is_floating_point<T>::value) return T(__builtin_inf());
+    else return std::numeric_limits<T>::max();
 }
 
 template<class T>
 __device__ inline T waveReduceMax(T v)
 {
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        const T o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    return v;
+    return lastLane(waveScanDpp(v, lowestOf<T>(), [](T a, T b) { return a > b ? a : b; }));
+}
+
+template<class T>
+__device__ inline T waveReduceMin(T v)
+{
+    return lastLane(waveScanDpp(v, highestOf<T>(), [](T a, T b) { return a < b ? a : b; }));
+}
+
+template<class T>
+__device__ inline T waveReduceOr(T v)
+{
+    return lastLane(waveScanDpp(v, T(0), [](T a, T b) { return a | b; }));
 }
 
 // Inclusive prefix sum over the wave.
 template<class T>
 __device__ inline T waveInclusiveScan(T v)
 {
-    const int lane = laneId();
-    for (int off = 1; off < 64; off <<= 1)
-    {
-        const T o = __shfl_up(v, off, 64);
-        if (lane >= off) v += o;
-    }
-    return v;
+    return waveScanDpp(v, T(0), [](T a, T b) { return a + b; });
+}
+
+// Inclusive prefix maximum over the wave.
+template<class T>
+__device__ inline T waveInclusiveMax(T v, T id)
+{
+    return waveScanDpp(v, id, [](T a, T b) { return a > b ? a : b; });
 }
 
 // Orders memory operations of one wavefront through LDS: what a lane stored is visible

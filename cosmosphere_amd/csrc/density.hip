@@ -89,6 +89,7 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
     using G = std::conditional_t<MIXED, float, double>;   // the gradient sums
     __shared__ int    segStart[kDensWaves][kWave];
     __shared__ int    segJ[kDensWaves][kWave];
+    __shared__ int    tileHead[kDensWaves][kWave];   // first tile position of each list entry (tileOwner)
     __shared__ double tj[kDensWaves][9][kWave];
     __shared__ double cp[kDensWaves][4][kWave];   // the chunk's particles: x, y, z, (R h)^2 slack
     __shared__ int    jq[kDensWaves][2 * kWave];  // candidates that passed the particle test
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4)
                 pair(sub + tt * g);
             }
         };
-        forEachTileCompact(segStart[w], segJ[w], jq[w], A.layout, A.jList, lo, hi, keep, keepJ, load, consume);
+        forEachTileCompact(segStart[w], segJ[w], tileHead[w], jq[w], A.layout, A.jList, lo, hi, keep, keepJ, load, consume);
         // group sums: a tree over lanes sub .. g - 1 of each group (lane sub = 0 ends with them)
         for (int off = 1; off < g; off <<= 1)
         {

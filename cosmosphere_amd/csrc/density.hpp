@@ -70,6 +70,7 @@ struct DensArgs
     int             closure;    // converged particles pass through state 2
     int             cull;       // whole-leaf items: stage only list entries some particle's support reaches
     int             units;      // mixed whole-leaf items: lanes over work units where that idles fewer lanes
+    int             tileDepth;  // device-driven pass: j-slots per LDS tile of whole-leaf items (64 or 128)
     unsigned long long* counters;
     PassCtl*        ctl;        // device-driven pass: persistent waves pull items from a queue
     const int*      qOrder;     // ... heavy items qOrder[0, ctl->qBucket[1]) first, then every item
@@ -217,7 +218,9 @@ int  counterStride();
 void launchForcePrep(const double* h, const double* gradh, const double* pro2, double* hsqinv, double* hinv,
                      double* rhoh, double* rho1, double* grkfac, double* pres, double* auterm, double* divfac, int n,
                      double pmass, double alphau, double hfact, hipStream_t st);
-void launchForce(const ForceArgs& a, bool periodic, bool disc, bool full, bool mixed, hipStream_t st);
+// rounds: the mixed form with each tile's accepted pairs shared evenly over the lanes
+// (forceKernelMixedRounds), else lanes that keep their own slots (forceKernelMixed)
+void launchForce(const ForceArgs& a, bool periodic, bool disc, bool full, bool mixed, bool rounds, hipStream_t st);
 // FP32 copies of the force factors for the mixed form (ForceArgs::f32 layout), n particles.
 void launchForceF32(const ForceArgs& a, int n, float* dst, hipStream_t st);
 void launchDeadForce(int first, int n, double* fx, double* fy, double* fz, double* f4, double* vsig, double* divv,

@@ -71,6 +71,7 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
 {
     __shared__ int    segStart[kForceWaves][kWave];
     __shared__ int    segJ[kForceWaves][kWave];
+    __shared__ int    tileHead[kForceWaves][kWave];   // first tile position of each list entry (tileOwner)
     // the tile: every j-field a pair reads, staged once per tile slot with coalesced loads
     // (a j is accepted by several particles of the chunk: reading its fields per pair would
     // be one scattered load per field per pair)
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
                     pair(sub + tt * g);
                 }
             };
-            forEachTileCompact(segStart[w], segJ[w], jq[w], A.layout, A.jList, lo, hi, keep, keepJ, load, consume);
+            forEachTileCompact(segStart[w], segJ[w], tileHead[w], jq[w], A.layout, A.jList, lo, hi, keep, keepJ, load, consume);
             // group sums (and the max of vsigmax): a tree over lanes sub .. g - 1 of each group
             for (int off = 1; off < g; off <<= 1)
             {
@@ -299,6 +300,7 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(4
 {
     __shared__ int    segStart[kForceWaves][kWave];
     __shared__ int    segJ[kForceWaves][kWave];
+    __shared__ int    tileHead[kForceWaves][kWave];   // first tile position of each list entry (tileOwner)
     // the tile: FP64 what enters a difference, FP32 the factors, all staged once per slot
     enum { X, Y, Z, VX, VY, VZ, U, ND };
     enum { H, HS, HI, GK, RH, R1, P2, CS, AL, PR, AU, NS };
@@ -490,7 +492,7 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(4
                 }
                 flush();
             };
-            forEachTileCompact(segStart[w], segJ[w], jq[w], A.layout, A.jList, lo, hi, keep, keepJ, load, consume);
+            forEachTileCompact(segStart[w], segJ[w], tileHead[w], jq[w], A.layout, A.jList, lo, hi, keep, keepJ, load, consume);
             double vm = (double)vmax;
             for (int off = 1; off < g; off <<= 1)
             {
@@ -516,6 +518,327 @@ __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(4
     }
     nAcc = waveReduceSum(nAcc);
     nCand = waveReduceSum(nCand);
+    if (laneId() == 0 && (nAcc | nCand))
+    {
+        unsigned long long* slot = A.counters + ((blockIdx.x * kForceWaves + w) % kCounterSlots) * kCounterStride;
+        atomicAdd(slot + 0, nAcc);
+        atomicAdd(slot + 1, nCand);
+    }
+}
+
+// A double moved from lane `src` (wave-uniform or not) to the calling lane, as two 32-bit pulls.
+__device__ inline double pullD(double v, int src)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = __shfl((int)x, src, kWave), hi = __shfl((int)(x >> 32), src, kWave);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
+}
+
+// A double of lane p as a scalar (p wave-uniform).
+__device__ inline double laneD(double v, int p)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)x, p), hi = __builtin_amdgcn_readlane((int)(x >> 32), p);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
+}
+
+// Position of the k-th (from 0) set bit of x, which has more than k set bits.
+__device__ inline int nthSetBit(uint64_t x, int k)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1)
+    {
+        const uint64_t lo = x & ((uint64_t(1) << w) - 1);
+        const int c = __popcll(lo);
+        if (k >= c)
+        {
+            k -= c;
+            x >>= w;
+            pos += w;
+        }
+        else
+        {
+            x = lo;
+        }
+    }
+    return pos;
+}
+
+// Mixed form with shared slots (the default): the same lists, culling, tile and pair terms as
+// forceKernelMixed, with the accepted pairs of a tile spread evenly over the wavefront.  Lane p
+// owns particle p of the chunk (up to 64 at a time).  For each tile:
+//   1. the support test of each particle runs over the 64 slots at once (its position read as
+//      scalars), giving its mask of accepted slots;
+//   2. with a_p accepted slots for particle p, the smallest m with sum ceil(a_p / m) <= 64 gives
+//      each particle ceil(a_p / m) consecutive lanes, each taking up to m of its accepted slots;
+//   3. each lane pulls its particle's factors from the owner lane once, evaluates its m pairs
+//      into FP32 partials, and adds them to the particle's partials in LDS (atomic adds within
+//      the wavefront, contended only by the lanes sharing the particle);
+//   4. the owners add the tile's partials into their FP64 sums.
+// A wavefront then runs the pair terms about total / 64 times per tile; with lanes that keep
+// their own slots it runs them as often as its busiest lane accepts a pair.
+template<bool PER, bool DISC, bool FULL>
+__global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArgs A)
+{
+    __shared__ int    segStart[kForceWaves][kWave];
+    __shared__ int    segJ[kForceWaves][kWave];
+    __shared__ int    tileHead[kForceWaves][kWave];   // staging (tileOwner); in consume, the lanes' particles
+    enum { X, Y, Z, VX, VY, VZ, U, ND };
+    enum { H, HS, HI, GK, RH, R1, P2, CS, AL, PR, AU, NS };
+    __shared__ double td[kForceWaves][ND][kWave];
+    __shared__ float  ts[kForceWaves][NS][kWave];
+    __shared__ double cp[kForceWaves][4][kWave];   // the chunk's particles: x, y, z, R h
+    __shared__ int    jq[kForceWaves][2 * kWave];  // candidates that passed the particle test
+    __shared__ uint64_t acceptMask[kForceWaves][kWave];   // per particle: its accepted slots of the tile
+    enum { AIX, AIY, AIZ, AJX, AJY, AJZ, AF4, ADIV, AVM, NA };
+    __shared__ float  pacc[kForceWaves][NA][kWave];   // the tile's partials per particle (AVM: the max's bits)
+    const int w = threadIdx.x >> 6, lane = laneId();
+    const size_t S = A.f32Stride;
+    auto F = [&](int k, uint32_t idx) { return A.f32[(size_t)k * S + idx]; };
+    int a;
+    uint32_t p0, p1;
+    unsigned long long nAcc = 0, nCand = 0;   // wave-uniform
+    if (A.items.resolve(blockIdx.x * kForceWaves + w, A.nLeaves, A.layout, a, p0, p1))
+    {
+        const int iLeaf = a;
+        const uint32_t lo = A.jOffset[a], hi = A.jOffset[a + 1];
+        const DBox box = A.boxes[A.leafSet[iLeaf]];
+        const float pm = (float)A.pmass, beta = (float)A.beta;
+        const float R2 = (float)kRad2;
+        for (uint32_t q0 = p0; q0 < p1; q0 += kWave)
+        {
+            const int c = __builtin_amdgcn_readfirstlane((int)min((uint32_t)kWave, p1 - q0));
+            const bool own = lane < c;
+            const uint32_t i = q0 + (uint32_t)(own ? lane : 0);
+            const double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+            const double vxi = A.vx[i], vyi = A.vy[i], vzi = A.vz[i], eni = A.u[i];
+            const float hif = (float)A.h[i];
+            const float hsi = F(0, i), hinvi = F(1, i), gki = F(2, i), rhoi = F(3, i), rho1i = F(4, i);
+            const float pro2i = F(5, i), vwi = F(6, i), ali = F(7, i), pri = F(8, i), aui = F(9, i);
+            double cbc[3], cbs[3], rChunk;   // the chunk's bounding box and largest reach R h
+            if (own)
+            {
+                cp[w][0][lane] = xi; cp[w][1][lane] = yi; cp[w][2][lane] = zi;
+                cp[w][3][lane] = kRad * A.h[i];
+            }
+            waveBox(own, xi, yi, zi, kRad * A.h[i], cbc, cbs, rChunk);
+#pragma unroll
+            for (int f = 0; f < NA; ++f) pacc[w][f][lane] = 0.0f;
+            waveLdsSync();
+            double ix = 0, iy = 0, iz = 0, jx = 0, jy = 0, jz = 0, f4s = 0, divs = 0;
+            float vmax = 0.0f;
+            auto keep = [&](int jl) {
+                double hj = A.hmaxLeaf[jl];
+                const double rj = (isfinite(hj) && hj > 0.0) ? kRad * hj : 0.0;
+                for (int k = 0; k < c; ++k)
+                {
+                    const double r = fmax(cp[w][3][k], rj);
+                    if (pointLeafGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.lbox, A.lboxN, jl, box) <=
+                        r * r * kCullSlack)
+                        return true;
+                }
+                return false;
+            };
+            auto keepJ = [&](int j) {
+                const double r = fmax(rChunk, kRad * A.h[j]);
+                return pointBoxGap2<PER>(A.x[j], A.y[j], A.z[j], cbc, cbs, box) <= r * r * kCullSlack;
+            };
+            auto load = [&](int l, int j) {
+                td[w][X][l] = A.x[j]; td[w][Y][l] = A.y[j]; td[w][Z][l] = A.z[j];
+                td[w][VX][l] = A.vx[j]; td[w][VY][l] = A.vy[j]; td[w][VZ][l] = A.vz[j]; td[w][U][l] = A.u[j];
+                ts[w][H][l] = (float)A.h[j];
+                ts[w][HS][l] = F(0, j); ts[w][HI][l] = F(1, j); ts[w][GK][l] = F(2, j); ts[w][RH][l] = F(3, j);
+                ts[w][R1][l] = F(4, j); ts[w][P2][l] = F(5, j); ts[w][CS][l] = F(6, j); ts[w][AL][l] = F(7, j);
+                ts[w][PR][l] = F(8, j); ts[w][AU][l] = F(9, j);
+            };
+            auto image = [&](double& dxd, double& dyd, double& dzd) {
+                if constexpr (PER)
+                {
+                    if (fabs(dxd) > 0.5 * box.l[0]) dxd = dxd - box.l[0] * copysign(1.0, dxd);
+                    if (fabs(dyd) > 0.5 * box.l[1]) dyd = dyd - box.l[1] * copysign(1.0, dyd);
+                    if (fabs(dzd) > 0.5 * box.l[2]) dzd = dzd - box.l[2] * copysign(1.0, dzd);
+                }
+            };
+            auto consume = [&](int nt) {
+                nCand += (unsigned long long)nt * (unsigned long long)c;
+                // 1. each particle's accepted slots (slot `lane` against particle p)
+                {
+                    const bool slot = lane < nt;
+                    const int sl = slot ? lane : 0;
+                    const double xj = td[w][X][sl], yj = td[w][Y][sl], zj = td[w][Z][sl];
+                    const float hj = ts[w][H][sl];
+                    const float hj2 = hj * hj;
+                    for (int p = 0; p < c; ++p)
+                    {
+                        // the pair test: 0 < r < R max(h_i, h_j)
+                        double dxd = laneD(xi, p) - xj, dyd = laneD(yi, p) - yj, dzd = laneD(zi, p) - zj;
+                        image(dxd, dyd, dzd);
+                        const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+                        const float dr2 = dx * dx + dy * dy + dz * dz;
+                        const float hp =
+                            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, hif), p));
+                        const float rmax2 = R2 * fmaxf(hp * hp, hj2) * (1.0f + 1e-6f);
+                        const uint64_t m = ballot64(slot && (dr2 > 0.0f) && (dr2 < rmax2));
+                        if (lane == 0) acceptMask[w][p] = m;
+                    }
+                }
+                waveLdsSync();
+                // 2. shares: m pairs per lane, ceil(a_p / m) lanes for particle p
+                const uint64_t mine = own ? acceptMask[w][lane] : 0;
+                const int ap = __popcll(mine);
+                const int total = waveReduceSum(ap);
+                if (total == 0) return;
+                int m = (total + kWave - 1) / kWave;
+                for (;;)
+                {
+                    const int need = waveReduceSum((ap + m - 1) / m);
+                    if (need <= kWave) break;
+                    ++m;
+                }
+                const int lanesP = (ap + m - 1) / m;
+                const int startP = waveInclusiveScan(lanesP) - lanesP;
+                tileHead[w][lane] = -1;
+                if (lanesP > 0) tileHead[w][startP] = lane;
+                waveLdsSync();
+                const int p = waveInclusiveMax(tileHead[w][lane], -1);   // this lane's particle (-1: none)
+                waveLdsSync();
+                const int pp = p < 0 ? 0 : p;
+                const int r = lane - __shfl(startP, pp, kWave);          // its share of the particle's pairs
+                const int apOf = __shfl(ap, pp, kWave);
+                const int cnt = p < 0 ? 0 : max(0, min(m, apOf - r * m));
+                uint64_t bits = 0;
+                if (cnt > 0)
+                {
+                    const uint64_t mk = acceptMask[w][pp];
+                    bits = mk & ~((uint64_t(1) << nthSetBit(mk, r * m)) - 1);
+                }
+                // 3. the particle's factors from its owner lane, then the lane's pairs
+                const double xp = pullD(xi, pp), yp = pullD(yi, pp), zp = pullD(zi, pp);
+                const double vxp = pullD(vxi, pp), vyp = pullD(vyi, pp), vzp = pullD(vzi, pp), enp = pullD(eni, pp);
+                const float hp = __shfl(hif, pp, kWave), hsp = __shfl(hsi, pp, kWave);
+                const float hinvp = __shfl(hinvi, pp, kWave), gkp = __shfl(gki, pp, kWave);
+                const float rhop = __shfl(rhoi, pp, kWave), rho1p = __shfl(rho1i, pp, kWave);
+                const float pro2p = __shfl(pro2i, pp, kWave), vwp = __shfl(vwi, pp, kWave);
+                const float alp = __shfl(ali, pp, kWave), prp = __shfl(pri, pp, kWave), aup = __shfl(aui, pp, kWave);
+                float tix = 0, tiy = 0, tiz = 0, tjx = 0, tjy = 0, tjz = 0, tf4 = 0, tdiv = 0, tvm = 0;
+                for (int t = 0; t < m; ++t)
+                {
+                    bool counted = false;
+                    if (t < cnt)
+                    {
+                        const int k = __builtin_ctzll(bits);
+                        bits &= bits - 1;
+                        double dxd = xp - td[w][X][k], dyd = yp - td[w][Y][k], dzd = zp - td[w][Z][k];
+                        image(dxd, dyd, dzd);
+                        const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+                        const float dr2 = dx * dx + dy * dy + dz * dz;
+                        const float hj = ts[w][H][k];
+                        const float qi2 = dr2 * hsp, qj2 = dr2 * ts[w][HS][k];
+                        const float dr = __builtin_amdgcn_sqrtf(dr2), rij1 = __builtin_amdgcn_rcpf(dr);
+                        const float rx = dx * rij1, ry = dy * rij1, rz = dz * rij1;
+                        const float dvx = (float)(vxp - td[w][VX][k]), dvy = (float)(vyp - td[w][VY][k]);
+                        const float dvz = (float)(vzp - td[w][VZ][k]);
+                        const float projv = dvx * rx + dvy * ry + dvz * rz;
+                        const float rhoj = ts[w][RH][k], rho1j = ts[w][R1][k], pro2j = ts[w][P2][k];
+                        const float vwj = ts[w][CS][k], alj = ts[w][AL][k], prj = ts[w][PR][k], auj = ts[w][AU][k];
+                        const float vsigi = fmaxf(vwp - beta * projv, 0.0f);
+                        const float vsigavi = fmaxf(alp * vwp - beta * projv, 0.0f);
+                        const float vsigj = fmaxf(vwj - beta * projv, 0.0f);
+                        const float vsigavj = fmaxf(alj * vwj - beta * projv, 0.0f);
+                        const float pvmax = fmaxf(vsigi, vsigj);
+                        float qr2i = 0.0f, qr2j = 0.0f;
+                        const float denij = (float)(enp - td[w][U][k]);
+                        const float vsigu =
+                            __builtin_amdgcn_sqrtf(fabsf(prp - prj) * (2.0f * __builtin_amdgcn_rcpf(rhop + rhoj)));
+                        if constexpr (DISC)
+                        {
+                            const float hori = hp * rij1, horj = hj * rij1;
+                            if (projv < 0.0f)
+                            {
+                                qr2i = -0.5f * rho1p * (alp * vwp - beta * projv) * hori * projv;
+                                qr2j = -0.5f * rho1j * (alj * vwj - beta * projv) * horj * projv;
+                            }
+                            else
+                            {
+                                qr2i = -0.5f * rho1p * alp * vwp * hori * projv;
+                                qr2j = -0.5f * rho1j * alj * vwj * horj * projv;
+                            }
+                        }
+                        else if (projv < 0.0f)
+                        {
+                            qr2i = -0.5f * rho1p * vsigavi * projv;
+                            qr2j = -0.5f * rho1j * vsigavj * projv;
+                        }
+                        if (qi2 < R2)
+                        {
+                            counted = true;
+                            tvm = fmaxf(tvm, pvmax);
+                            float wk, dwk;
+                            const float q = dr * hinvp;
+                            kernWdW<float>(qi2, q, wk, dwk);
+                            tdiv += pm * dwk * projv;
+                            const float gk = dwk * gkp;
+                            const float gp = pm * (pro2p + qr2i) * gk;
+                            tix -= gp * rx; tiy -= gp * ry; tiz -= gp * rz;
+                            const float pdvt = pm * pro2p * projv * gk;
+                            const float diss = DISC ? -0.5f * pm * rho1p * alp * vwp * (hp * rij1) * projv * projv * gk
+                                                    : pm * qr2i * projv * gk;
+                            const float dend = vsigu * denij * aup * gk;
+                            if constexpr (FULL) { tf4 += pdvt; tf4 += diss; }
+                            else
+                            {
+                                if (A.pdv) tf4 += pdvt;
+                                if (A.shock) tf4 += diss;
+                            }
+                            tf4 += dend;
+                        }
+                        if (qj2 < R2)
+                        {
+                            counted = true;
+                            tvm = fmaxf(tvm, pvmax);
+                            float wk, dwk;
+                            const float q = dr * ts[w][HI][k];
+                            kernWdW<float>(qj2, q, wk, dwk);
+                            const float gk = dwk * ts[w][GK][k];
+                            const float gp = pm * (pro2j + qr2j) * gk;
+                            tjx -= gp * rx; tjy -= gp * ry; tjz -= gp * rz;
+                            tf4 += vsigu * denij * auj * gk;
+                        }
+                    }
+                    nAcc += (unsigned long long)__popcll(ballot64(counted));
+                }
+                // 4. the lane's partials into its particle's
+                if (cnt > 0)
+                {
+                    atomicAdd(&pacc[w][AIX][pp], tix); atomicAdd(&pacc[w][AIY][pp], tiy);
+                    atomicAdd(&pacc[w][AIZ][pp], tiz); atomicAdd(&pacc[w][AJX][pp], tjx);
+                    atomicAdd(&pacc[w][AJY][pp], tjy); atomicAdd(&pacc[w][AJZ][pp], tjz);
+                    atomicAdd(&pacc[w][AF4][pp], tf4); atomicAdd(&pacc[w][ADIV][pp], tdiv);
+                    atomicMax(reinterpret_cast<unsigned*>(&pacc[w][AVM][pp]), __float_as_uint(tvm) & 0x7FFFFFFFu);
+                }
+                waveLdsSync();
+                if (own)
+                {
+                    ix += (double)pacc[w][AIX][lane]; iy += (double)pacc[w][AIY][lane]; iz += (double)pacc[w][AIZ][lane];
+                    jx += (double)pacc[w][AJX][lane]; jy += (double)pacc[w][AJY][lane]; jz += (double)pacc[w][AJZ][lane];
+                    f4s += (double)pacc[w][AF4][lane]; divs += (double)pacc[w][ADIV][lane];
+                    vmax = fmaxf(vmax, pacc[w][AVM][lane]);
+                }
+#pragma unroll
+                for (int f = 0; f < NA; ++f) pacc[w][f][lane] = 0.0f;
+                waveLdsSync();
+            };
+            forEachTileCompact(segStart[w], segJ[w], tileHead[w], jq[w], A.layout, A.jList, lo, hi, keep, keepJ, load,
+                               consume);
+            if (own)
+            {
+                A.fx[i] = ix + jx; A.fy[i] = iy + jy; A.fz[i] = iz + jz;
+                A.f4[i] = f4s; A.vsig[i] = (double)vmax;
+                A.divv[i] = -divs * A.divfac[i];
+            }
+        }
+    }
     if (laneId() == 0 && (nAcc | nCand))
     {
         unsigned long long* slot = A.counters + ((blockIdx.x * kForceWaves + w) % kCounterSlots) * kCounterStride;
@@ -561,14 +884,15 @@ void launchForceF32(const ForceArgs& a, int n, float* dst, hipStream_t st)
     LAUNCH_CHECK();
 }
 
-void launchForce(const ForceArgs& a, bool periodic, bool disc, bool full, bool mixed, hipStream_t st)
+void launchForce(const ForceArgs& a, bool periodic, bool disc, bool full, bool mixed, bool rounds, hipStream_t st)
 {
     if (a.nLeaves <= 0) return;
     const int grid = iceil(a.items.count(a.nLeaves), kForceWaves);
-#define CSPH_F(P, D, F)                                                             \
-    do {                                                                            \
-        if (mixed) forceKernelMixed<P, D, F><<<grid, kForceThreads, 0, st>>>(a);    \
-        else       forceKernel<P, D, F><<<grid, kForceThreads, 0, st>>>(a);         \
+#define CSPH_F(P, D, F)                                                                           \
+    do {                                                                                          \
+        if (mixed && rounds) forceKernelMixedRounds<P, D, F><<<grid, kForceThreads, 0, st>>>(a);  \
+        else if (mixed)      forceKernelMixed<P, D, F><<<grid, kForceThreads, 0, st>>>(a);        \
+        else                 forceKernel<P, D, F><<<grid, kForceThreads, 0, st>>>(a);             \
     } while (0)
     if (periodic)
     {

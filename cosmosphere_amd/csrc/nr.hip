@@ -330,28 +330,37 @@ __device__ inline void particleSums(double s0, double s1, double eps, double& ws
 // PP: the launch holds per-particle items (a separate instance keeps the register budget of
 // the whole-leaf path low when it does not).
 // QUEUE: the device-driven pass's persistent grid (A.ctl), a separate instance so the
-// register budget of the host-driven kernels is unchanged; held to 80 VGPRs (6 waves per SIMD).
+// register budget of the host-driven kernels is unchanged; held to 5 waves per SIMD (up to 96
+// VGPRs: the instances need 87-91 and do not spill; at 6 waves, 80 VGPRs, they spilled 14-69
+// registers and the 8M-particle Plummer mixture solved 0.5% (mixed) and 2% (FP64) slower).
 // CULL: whole-leaf items stage only the list entries some particle's own support reaches (a
 // separate instance: the culling table and test cost registers the plain instances keep).
-template<class Real, bool PER, int MODE, bool PP, bool QUEUE = false, bool CULL = false>
-__global__ __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(QUEUE ? 6 : 1)))
+#ifndef COSMO_QUEUE_WAVES
+#define COSMO_QUEUE_WAVES 5
+#endif
+// TD: slots per LDS tile of the whole-leaf items (the tile depth, 64 or 128; a tuning option of
+// the device-driven loop).
+template<class Real, bool PER, int MODE, bool PP, bool QUEUE = false, bool CULL = false, int TD = kWave>
+__global__ __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(QUEUE ? COSMO_QUEUE_WAVES : 1)))
 void densityKernel(DensArgs A)
 {
     constexpr bool kMixed = sizeof(Real) == 4;
     __shared__ int segStart[kNrWaves][kWave];
     __shared__ int segJ[kNrWaves][kWave];
+    __shared__ int    tileHead[kNrWaves][kWave];   // first tile position of each list entry (tileOwner)
     __shared__ int rankLane[kNrWaves][kWave];   // whole-leaf items: k-th particle to evaluate -> lane
     __shared__ double cpw[CULL ? kNrWaves : 1][4][CULL ? kWave : 1];   // ... its x, y, z, (R h)^2 slack
     // whole-leaf tiles: mixed = fixed-point offsets (x, y, z, pad), one 16-byte LDS read per j;
     // plain = FP64 positions
-    __shared__ int4   tq[kMixed ? kNrWaves : 1][kMixed ? 2 * kWave : 1];
+    // (TD slots and the padding read beyond them: TD + 64)
+    __shared__ int4   tq[kMixed ? kNrWaves : 1][kMixed ? TD + kWave : 1];
     // mixed whole-leaf items in work units: the batch's particles (fixed-point offsets, hsr bits)
     __shared__ int4   ptab[kMixed ? kNrWaves : 1][kMixed ? kWave : 1];
     // plain whole-leaf items in work units (culling instances): positions from cpw, 1/h^2 here
     __shared__ double ptabH[(!kMixed && CULL) ? kNrWaves : 1][(!kMixed && CULL) ? kWave : 1];
-    __shared__ double tx[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
-    __shared__ double ty[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
-    __shared__ double tz[kMixed ? 1 : kNrWaves][kMixed ? 1 : 2 * kWave];
+    __shared__ double tx[kMixed ? 1 : kNrWaves][kMixed ? 1 : TD + kWave];
+    __shared__ double ty[kMixed ? 1 : kNrWaves][kMixed ? 1 : TD + kWave];
+    __shared__ double tz[kMixed ? 1 : kNrWaves][kMixed ? 1 : TD + kWave];
 
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = laneId();
     unsigned long long ctr[4] = {0, 0, 0, 0};   // NR pairs, NR candidates, closure pairs, closure candidates
@@ -401,7 +410,7 @@ void densityKernel(DensArgs A)
                 }
                 ++cand;
             };
-            forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, pair, [](int) {}, [](int) {});
+            forEachTileIf(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, pair, [](int) {}, [](int) {});
             s0 = waveReduceSum(s0);
             s1 = waveReduceSum(s1);
             const int slot = isClosureEval<MODE>(st) ? 2 : 0;
@@ -485,9 +494,12 @@ void densityKernel(DensArgs A)
                 // tiles keep slot order (position = slot), which measured 3% faster than sweep
                 // order there.
                 const int Sl = U > 1 ? (int)shapes.S[c] : g;
-                const int Tl = (kWave + Sl - 1) / Sl;
-                const int pos = kMixed ? lane : (lane % Sl) * Tl + lane / Sl;   // of slot `lane`, staged by this lane
-                auto load = [&](int, int j) {
+                const int Tl = (TD + Sl - 1) / Sl;
+                // positions of the slots this lane stages: `lane` and, with 128-slot tiles, 64 + lane
+                const int pos0 = kMixed ? lane : (lane % Sl) * Tl + lane / Sl;
+                const int pos1 = kMixed ? kWave + lane : ((kWave + lane) % Sl) * Tl + (kWave + lane) / Sl;
+                auto load = [&](int slot, int j) {
+                    const int pos = slot < kWave ? pos0 : pos1;
                     double xj = A.x[j], yj = A.y[j], zj = A.z[j];
                     if constexpr (kMixed)
                     {
@@ -506,13 +518,13 @@ void densityKernel(DensArgs A)
                     if constexpr (kMixed) tq[w][k] = make_int4(kFarFixed, kFarFixed, kFarFixed, 0);
                     else { tx[w][k] = 1e200; ty[w][k] = 1e200; tz[w][k] = 1e200; }
                 };
-                auto pad = [&](int) { padAt(pos); };
+                auto pad = [&](int slot) { padAt(slot < kWave ? pos0 : pos1); };
                 // the tiles stage slots 0 .. 63; the slots a sweep reads beyond them hold padding
                 // through the batch: mixed, slots below 2 S + 63 <= 127 (the strided sweeps with
                 // their read-ahead); FP64, slots below Sl Tl <= 95 (sweep order, the read-ahead
                 // reaches position Sl Tl <= 127 at most)
-                if constexpr (kMixed) padAt(kWave + lane);
-                else if (lane < Sl * Tl - kWave) padAt(((kWave + lane) % Sl) * Tl + (kWave + lane) / Sl);
+                if constexpr (kMixed) padAt(TD + lane);
+                else if (lane < Sl * Tl - TD) padAt(((TD + lane) % Sl) * Tl + (TD + lane) / Sl);
                 if constexpr (kMixed || CULL)
                 {
                     if (U > 1)
@@ -585,7 +597,7 @@ void densityKernel(DensArgs A)
                                 }
                             }
                         };
-                        forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, pad, consumeUnits);
+                        forEachTileIf<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, load, pad, consumeUnits);
                         // counters by the state of each lane's particles, read before any update
                         if (u0 < nU)
                         {
@@ -705,7 +717,7 @@ void densityKernel(DensArgs A)
                         }
                         cand += sub == 0 ? (uint32_t)nt : 0u;
                     };
-                    forEachTileIf(segStart[w], segJ[w], A.layout, A.jList, lo, hi, keep, load, pad, consume);
+                    forEachTileIf<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, load, pad, consume);
                     if (has)
                     {
                         const int slot = isClosureEval<MODE>(st) ? 2 : 0;
@@ -914,18 +926,17 @@ void launchDensity(const DensArgs& a, int precision, int mode, bool periodic, hi
 
 void launchDensityPersistent(const DensArgs& a, int precision, bool periodic, int blocks, hipStream_t st)
 {
-#define CSPH_DENS_P(R, P)                                                             \
-    do {                                                                              \
-        if (a.cull)                                                                   \
-        {                                                                             \
-            densityKernel<R, P, 0, false, true, true><<<blocks, kNrThreads, 0, st>>>(a); \
-            densityKernel<R, P, 0, true, true, true><<<blocks, kNrThreads, 0, st>>>(a);  \
-        }                                                                             \
-        else                                                                          \
-        {                                                                             \
-            densityKernel<R, P, 0, false, true><<<blocks, kNrThreads, 0, st>>>(a);       \
-            densityKernel<R, P, 0, true, true><<<blocks, kNrThreads, 0, st>>>(a);        \
-        }                                                                             \
+#define CSPH_DENS_T(R, P, C, T)                                                                   \
+    do {                                                                                          \
+        densityKernel<R, P, 0, false, true, C, T><<<blocks, kNrThreads, 0, st>>>(a);              \
+        densityKernel<R, P, 0, true, true, C, T><<<blocks, kNrThreads, 0, st>>>(a);               \
+    } while (0)
+#define CSPH_DENS_P(R, P)                                                                         \
+    do {                                                                                          \
+        if (a.cull) { if (a.tileDepth == 2 * kWave) CSPH_DENS_T(R, P, true, 2 * kWave);           \
+                      else CSPH_DENS_T(R, P, true, kWave); }                                      \
+        else        { if (a.tileDepth == 2 * kWave) CSPH_DENS_T(R, P, false, 2 * kWave);          \
+                      else CSPH_DENS_T(R, P, false, kWave); }                                     \
     } while (0)
     if (precision == kMixed)
     {
@@ -938,6 +949,7 @@ void launchDensityPersistent(const DensArgs& a, int precision, bool periodic, in
         else          CSPH_DENS_P(double, false);
     }
 #undef CSPH_DENS_P
+#undef CSPH_DENS_T
     LAUNCH_CHECK();
 }
 

@@ -155,7 +155,7 @@ __global__ void keySpreadKernel(const uint64_t* __restrict__ keys, int n, unsign
     const uint64_t k0 = keys[0];
     uint64_t acc = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc |= keys[i] ^ k0;
-    for (int off = 32; off > 0; off >>= 1) acc |= __shfl_xor(acc, off, 64);
+    acc = waveReduceOr(acc);
     if (laneId() == 0 && acc) atomicOr(out, (unsigned long long)acc);
 }
 

@@ -12,8 +12,8 @@
  *     the previous tree (and that leaf held at most 64 particles).  The previous leaves are
  *     disjoint key ranges in ascending order, and their particles are contiguous in the
  *     input, so the stayers, sorted within each previous leaf, form one sorted sequence A.
- *  2. One wavefront per previous leaf ranks its stayers by (key, caller index) with
- *     cross-lane broadcasts and writes them to their place in A.
+ *  2. One wavefront per previous leaf ranks its stayers by (key, caller index) against each
+ *     lane's entry read as a scalar, and writes them to their place in A.
  *  3. The movers (new key outside their leaf's range, dead particles, leaves of more than 64
  *     particles) are compacted and radix-sorted: by caller index, then stably by key.
  *  4. A merge-path merge of A and the sorted movers writes the sorted keys, the input
@@ -95,11 +95,11 @@ __global__ __launch_bounds__(256) void staySortKernel(const uint64_t* __restrict
                                                       uint64_t* __restrict__ wKey, uint32_t* __restrict__ wCal,
                                                       uint32_t* __restrict__ wIdx)
 {
-    const int l = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int l = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
     if (l >= oldLeaves) return;
     const int lane = threadIdx.x % kWave;
-    const int a = (int)oldLayout[l];
-    const int c = (int)oldLayout[l + 1] - a;
+    const int a = __builtin_amdgcn_readfirstlane((int)oldLayout[l]);
+    const int c = __builtin_amdgcn_readfirstlane((int)oldLayout[l + 1] - a);
     if (c > kSegMax) return;   // flagged as movers
     const int i = a + lane;
     uint64_t k = kDeadKey;
@@ -114,9 +114,11 @@ __global__ __launch_bounds__(256) void staySortKernel(const uint64_t* __restrict
     int rank = 0;
     for (int t = 0; t < c; ++t)
     {
-        const uint64_t kt = ((uint64_t)(uint32_t)__shfl((int)(k >> 32), t) << 32) | (uint32_t)__shfl((int)(uint32_t)k, t);
-        const uint32_t ct = (uint32_t)__shfl((int)cal, t);
-        const int stt = __shfl(st, t);
+        // t is wave-uniform: lane t's entry as scalars
+        const uint64_t kt = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(k >> 32), t) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, t);
+        const uint32_t ct = (uint32_t)__builtin_amdgcn_readlane((int)cal, t);
+        const int stt = __builtin_amdgcn_readlane(st, t);
         rank += (stt && pairLess(kt, ct, k, cal)) ? 1 : 0;
     }
     if (lane < c && st)

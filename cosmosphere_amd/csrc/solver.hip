@@ -134,6 +134,7 @@ static DensArgs densArgs(State& s, const int* leaves, int nLeaves, double pmass,
     a.nItems = nLeaves;   // whole-leaf items unless the caller sets the item arrays
     a.cull = s.cull;
     a.units = s.units;
+    a.tileDepth = s.tileDepth;
     return a;
 }
 
@@ -442,7 +443,7 @@ ForceTimings computeForces(State& s, double pmass, double beta, double alphau, b
         a.f32 = f32;
     }
     a.items = buildSweepItems(s, a.nLeaves, s.sweepSplit, st);
-    launchForce(a, per, disc, pdv && shock, mixed, st);
+    launchForce(a, per, disc, pdv && shock, mixed, s.forceRounds != 0, st);
     launchDeadForce(na, n, s.fx.as<double>(), s.fy.as<double>(), s.fz.as<double>(), s.f4.as<double>(),
                     s.vsig.as<double>(), s.divvF.as<double>(), st);
     e3.rec(st);

@@ -185,12 +185,14 @@ struct State
     int      lpt = 1;              // work queue: the heaviest items first
     int      cull = 1;             // NR whole-leaf items: per-particle culling of list entries
     int      units = 1;            // mixed NR whole-leaf items: lanes over work units where that idles fewer lanes
+    int      tileDepth = 128;      // device-driven NR passes: j-slots per LDS tile of whole-leaf items (64 or 128)
     int      loopBlockScale = 4;   // persistent density grid: resident blocks x scale / 4
     int      loopBlocks = 0, loopBlocksPrecision = -1, loopBlocksPeriodic = -1;
     long long loopListCap = 0;     // verification: starting list capacity of the device-driven loop (0: auto)
     int listHeavySteps = 0;        // verification: walk steps before a list goes to the block walk (0: default)
     int listExpandLeaves = -1;     // verification: block walk expands nodes above this many leaves (-1: default)
     int splitLen = -1;             // device-driven loop: heavy items with longer lists are split (-1: default, 0: off)
+    int forceRounds = 0;           // mixed force pass: 1 shares each tile's accepted pairs over the lanes, 0 lanes keep their own slots
     int sweepSplit = 512;          // gradient / force sweeps: leaves with more list entries go one particle per wavefront (0: off)
     std::vector<PassRecord> passLog;   // one record per NR pass of the last solve
     int      closureOn = -1;   // -1: on for mixed, off for plain

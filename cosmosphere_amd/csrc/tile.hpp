@@ -12,14 +12,34 @@ namespace csph
 constexpr int kCounterSlots  = 256;
 constexpr int kCounterStride = 16;   // u64 per slot: one 128-byte line each
 
-// Stage the j-particles of list entries [lo, hi) as 64-slot tiles, skipping the entries
-// whose leaf keep(jl) rejects.  For every tile, each lane with a slot calls load(slot, j) and
-// each lane without one pad(slot); then consume(nt) runs with the tile complete.
-template<class Keep, class Load, class Pad, class Consume>
-__device__ inline void forEachTileIf(int* __restrict__ segStart, int* __restrict__ segJ,
+// The list entry owning slot s0 + lane of a tile.  Entry t (lane t) owns slots [start, start +
+// count) of the batch (start: exclusive prefix sum of the counts).  Each entry overlapping the
+// tile writes its index at its first tile position in `head` (distinct positions: two entries
+// with slots cannot start at the same one), and an inclusive prefix maximum over the lanes
+// carries it to the entry's other slots.  Lanes past the batch's total get -1 or a stale entry;
+// the callers use the owner of slots below the total only.
+__device__ inline int tileOwner(int* __restrict__ head, int s0, int start, int count)
+{
+    const int lane = laneId();
+    head[lane] = -1;
+    if (count > 0 && start < s0 + kWave && start + count > s0) head[max(start, s0) - s0] = lane;
+    waveLdsSync();
+    const int own = waveInclusiveMax(head[lane], -1);
+    waveLdsSync();   // head is rewritten for the next tile
+    return own;
+}
+
+// Stage the j-particles of list entries [lo, hi) as tiles of TD slots (64 or 128: the tile
+// depth), skipping the entries whose leaf keep(jl) rejects.  For every tile, each lane stages
+// TD / 64 slots: load(slot, j) for a slot with a particle, pad(slot) for one past the end;
+// then consume(nt) runs with the tile complete.  A deeper tile halves the consume calls (their
+// per-tile setup) against twice the LDS.
+template<int TD = kWave, class Keep, class Load, class Pad, class Consume>
+__device__ inline void forEachTileIf(int* __restrict__ segStart, int* __restrict__ segJ, int* __restrict__ head,
                                      const uint32_t* __restrict__ layout, const int* __restrict__ jList, uint32_t lo,
                                      uint32_t hi, Keep keep, Load load, Pad pad, Consume consume)
 {
+    static_assert(TD == kWave || TD == 2 * kWave, "tile depth 64 or 128");
     const int lane = laneId();
     for (uint32_t e0 = lo; e0 < hi; e0 += kWave)
     {
@@ -35,31 +55,30 @@ __device__ inline void forEachTileIf(int* __restrict__ segStart, int* __restrict
             }
         }
         const int incl  = waveInclusiveScan(jc);
-        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));   // wave-uniform
+        const int total = lastLane(incl);   // wave-uniform
         segStart[lane] = incl - jc;
         segJ[lane]     = js;
-        const int nseg = (int)min((uint32_t)kWave, hi - e0);
         waveLdsSync();
-        for (int s0 = 0; s0 < total; s0 += kWave)
+        for (int s0 = 0; s0 < total; s0 += TD)
         {
-            const int slot = s0 + lane;
-            if (slot < total)
+#pragma unroll
+            for (int part = 0; part < TD / kWave; ++part)
             {
-                int a = 0, b = nseg;   // last t with segStart[t] <= slot
-                while (b - a > 1)
+                const int base = s0 + part * kWave;   // wave-uniform
+                if (base < total)
                 {
-                    const int m = (a + b) >> 1;
-                    if (segStart[m] <= slot) a = m;
-                    else                     b = m;
+                    const int slot = base + lane;
+                    const int a = tileOwner(head, base, incl - jc, jc);
+                    if (slot < total) load(part * kWave + lane, segJ[a] + slot - segStart[a]);
+                    else              pad(part * kWave + lane);
                 }
-                load(lane, segJ[a] + slot - segStart[a]);
-            }
-            else
-            {
-                pad(lane);
+                else
+                {
+                    pad(part * kWave + lane);
+                }
             }
             waveLdsSync();
-            consume(__builtin_amdgcn_readfirstlane(min(kWave, total - s0)));
+            consume(__builtin_amdgcn_readfirstlane(min(TD, total - s0)));
             waveLdsSync();
         }
     }
@@ -71,7 +90,8 @@ __device__ inline void forEachTileIf(int* __restrict__ segStart, int* __restrict
 // more for the rest at the end) load(slot, j) runs for every slot, then consume(nt).  A
 // particle-level test that rejects most of a leaf's particles then costs no pair sweeps.
 template<class Keep, class KeepJ, class Load, class Consume>
-__device__ inline void forEachTileCompact(int* __restrict__ segStart, int* __restrict__ segJ, int* __restrict__ queue,
+__device__ inline void forEachTileCompact(int* __restrict__ segStart, int* __restrict__ segJ, int* __restrict__ head,
+                                          int* __restrict__ queue,
                                           const uint32_t* __restrict__ layout, const int* __restrict__ jList,
                                           uint32_t lo, uint32_t hi, Keep keep, KeepJ keepJ, Load load,
                                           Consume consume)
@@ -92,26 +112,16 @@ __device__ inline void forEachTileCompact(int* __restrict__ segStart, int* __res
             }
         }
         const int incl  = waveInclusiveScan(jc);
-        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));   // wave-uniform
+        const int total = lastLane(incl);   // wave-uniform
         segStart[lane] = incl - jc;
         segJ[lane]     = js;
-        const int nseg = (int)min((uint32_t)kWave, hi - e0);
         waveLdsSync();
         for (int s0 = 0; s0 < total; s0 += kWave)
         {
             const int slot = s0 + lane;
+            const int a = tileOwner(head, s0, incl - jc, jc);
             int j = -1;
-            if (slot < total)
-            {
-                int a = 0, b = nseg;   // last t with segStart[t] <= slot
-                while (b - a > 1)
-                {
-                    const int m = (a + b) >> 1;
-                    if (segStart[m] <= slot) a = m;
-                    else                     b = m;
-                }
-                j = segJ[a] + slot - segStart[a];
-            }
+            if (slot < total) j = segJ[a] + slot - segStart[a];
             const bool ok = j >= 0 && keepJ(j);
             const uint64_t m = ballot64(ok);
             if (ok) queue[fill + __popcll(m & lanemaskLt())] = j;   // < 2 kWave
@@ -155,16 +165,13 @@ __device__ inline void waveBox(bool own, double x, double y, double z, double r,
         hi[a] = own ? p[a] : -__builtin_inf();
     }
     double rm = own ? r : 0.0;
-    for (int off = 1; off < kWave; off <<= 1)
-    {
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-        {
-            lo[a] = fmin(lo[a], __shfl_xor(lo[a], off, kWave));
-            hi[a] = fmax(hi[a], __shfl_xor(hi[a], off, kWave));
-        }
-        rm = fmax(rm, __shfl_xor(rm, off, kWave));
+    for (int a = 0; a < 3; ++a)
+    {
+        lo[a] = waveReduceMin(lo[a]);
+        hi[a] = waveReduceMax(hi[a]);
     }
+    rm = waveReduceMax(rm);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
     {
@@ -217,11 +224,11 @@ __device__ inline double pointLeafGap2(double px, double py, double pz, const do
 }
 
 template<class Load, class Consume>
-__device__ inline void forEachTile(int* __restrict__ segStart, int* __restrict__ segJ,
+__device__ inline void forEachTile(int* __restrict__ segStart, int* __restrict__ segJ, int* __restrict__ head,
                                    const uint32_t* __restrict__ layout, const int* __restrict__ jList, uint32_t lo,
                                    uint32_t hi, Load load, Consume consume)
 {
-    forEachTileIf(segStart, segJ, layout, jList, lo, hi, [](int) { return true; }, load, [](int) {}, consume);
+    forEachTileIf(segStart, segJ, head, layout, jList, lo, hi, [](int) { return true; }, load, [](int) {}, consume);
 }
 
 } // namespace csph

@@ -54,11 +54,8 @@ __global__ __launch_bounds__(256) void bboxPartialKernel(const double* __restric
 #pragma unroll
     for (int a = 0; a < 3; ++a)
     {
-        for (int off = 32; off > 0; off >>= 1)
-        {
-            mn[a] = fmin(mn[a], __shfl_xor(mn[a], off, 64));
-            mx[a] = fmax(mx[a], __shfl_xor(mx[a], off, 64));
-        }
+        mn[a] = waveReduceMin(mn[a]);
+        mx[a] = waveReduceMax(mx[a]);
     }
     const int w = threadIdx.x / kWave;
     if (laneId() == 0)
@@ -450,7 +447,7 @@ __global__ __launch_bounds__(256) void leafBoxKernel(const double* __restrict__ 
             lo = fmin(lo, v);
             hi = fmax(hi, v);
         }
-        lo = -waveReduceMax(-lo);
+        lo = waveReduceMin(lo);
         hi = waveReduceMax(hi);
         if (laneId() == 0)
         {

@@ -15,6 +15,7 @@
  *   lpt           heaviest items first in the work queue
  *   cull          per-particle culling of the NR whole-leaf items' list entries
  *   units         mixed whole-leaf items: lanes over work units (fewer idle lanes)
+ *   tileDepth     device loop: 64 or 128 j-slots per LDS tile (per-tile setup vs LDS, occupancy)
  *
  * by a coordinate search: one option at a time, each candidate value timed by a solve (a second
  * solve confirms a candidate within 5% of the best), the best kept, sweeps repeated until
@@ -22,6 +23,7 @@
  * signature for the life of the process, and in the file named by COSMO_TUNE_CACHE if set.
  */
 #include "tune.hpp"
+#include "xfer.hpp"
 
 #include <chrono>
 #include <cstdio>
@@ -37,7 +39,7 @@ namespace csph
 TuneParams currentParams(const State& s)
 {
     return TuneParams{s.loopMode, s.bucket, s.mapping, s.sparse, s.tailLeaves, s.loopBlockScale, s.graphPasses, s.lpt,
-                      s.cull, s.units};
+                      s.cull, s.units, s.tileDepth};
 }
 
 void applyParams(State& s, const TuneParams& p)
@@ -52,24 +54,25 @@ void applyParams(State& s, const TuneParams& p)
     s.lpt = p.lpt;
     s.cull = p.cull;
     s.units = p.units;
+    s.tileDepth = p.tileDepth;
 }
 
 std::string describe(const TuneParams& p)
 {
     char b[256];
     std::snprintf(b, sizeof(b), "loop=%d bucket=%d mapping=%d sparse=%d tail_leaves=%d loop_blocks_scale=%d "
-                                "graph_passes=%d lpt=%d cull=%d units=%d",
+                                "graph_passes=%d lpt=%d cull=%d units=%d tile_depth=%d",
                   p.loop, p.bucket, p.mapping, p.sparse, p.tailLeaves, p.blockScale, p.graphPasses, p.lpt, p.cull,
-                  p.units);
+                  p.units, p.tileDepth);
     return b;
 }
 
 static bool parse(const std::string& text, TuneParams& p)
 {
     return std::sscanf(text.c_str(), "loop=%d bucket=%d mapping=%d sparse=%d tail_leaves=%d loop_blocks_scale=%d "
-                                     "graph_passes=%d lpt=%d cull=%d units=%d",
+                                     "graph_passes=%d lpt=%d cull=%d units=%d tile_depth=%d",
                        &p.loop, &p.bucket, &p.mapping, &p.sparse, &p.tailLeaves, &p.blockScale, &p.graphPasses,
-                       &p.lpt, &p.cull, &p.units) == 10;
+                       &p.lpt, &p.cull, &p.units, &p.tileDepth) == 11;
 }
 
 namespace
@@ -121,6 +124,7 @@ struct Inputs
     }
     void save()
     {
+        inputsToCallerOrder(s);   // the trials' solves read their inputs as the caller's order
         double* d = keep.get<double>((size_t)10 * n + 1);
         for (int k = 0; k < 10; ++k)
             HIP_CHECK(hipMemcpyAsync(d + (size_t)k * n, field(k)->as<double>(), (size_t)n * 8, hipMemcpyDeviceToDevice,
@@ -201,6 +205,7 @@ TuneResult autotune(State& s, const std::string& signature, const std::function<
         {&TuneParams::lpt, {1, 0}, deviceLoopOn},
         {&TuneParams::cull, {0, 1}, always},
         {&TuneParams::units, {1, 0}, always},
+        {&TuneParams::tileDepth, {64, 128}, deviceLoopOn},
     };
     bool improved = true;
     for (int sweep = 0; sweep < 3 && improved && elapsed() < budgetSeconds; ++sweep)

@@ -25,6 +25,7 @@ struct TuneParams
     int lpt;           // device loop: heaviest items first in the work queue
     int cull;          // NR whole-leaf items: per-particle culling of list entries
     int units;         // mixed NR whole-leaf items: lanes over work units
+    int tileDepth;     // device loop: j-slots per LDS tile of whole-leaf items (64 or 128)
 };
 
 TuneParams currentParams(const State& s);

@@ -101,6 +101,27 @@ def test_split_items(name, precision, mapping):
     assert np.array_equal(dev.array("h"), h_host)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["plummer_quintic", "C1small", "dead"])
+def test_tile_depths(name, precision):
+    """64- and 128-slot j-tiles (an option of the device-driven loop; 128 is the default) sum
+    each particle's pairs in other lane orders: the same pairs (counters equal), h and rho
+    against the oracle (FP64 1e-10, mixed 1e-6), the same iteration count."""
+    c = dataset(name)
+    print("case tile depth", name, precision, flush=True)
+    shallow = solve(c, precision, loop=1, tile_depth=64)
+    deep = solve(c, precision, loop=1, tile_depth=128)
+    o, unconv = oracle_run(c, gradients=False, max_iter=c.get("max_iter", 100))
+    assert unconv == 0
+    na = o.geti("nAlive")
+    if precision == 0:   # (the mixed form may stop a particle one iteration apart)
+        assert deep["iters"] == shallow["iters"] == o.geti("itersRun")
+        assert deep["pairs"] == shallow["pairs"] and deep["candidates"] == shallow["candidates"]
+    tol = 1e-10 if precision == 0 else 1e-6
+    for out in (shallow, deep):
+        assert rel_err(out["h"][:na], o.get("h")[:na]) < tol
+
+
 def test_device_loop_oracle_and_graph_reuse():
     c = dataset("plummer_quintic")
     lib, iters = gpu_run(c, precision=0, gradients=False)

@@ -134,6 +134,8 @@ def test_gradient_sweep(name, sweep, precision):
         for f in ("divv", "ddivvdt"):
             ref = host(f)
             errs[f] = np.max(np.abs(out[f] - ref)) / np.abs(ref).max()
+            # per particle, relative to its own value with a floor of 1e-3 of the largest
+            errs[f + "_pp"] = rel_err(out[f], ref, floor=np.abs(ref).max() * 1e-3)
         errs["xi"] = np.max(np.abs(out["xi"] - host("xi")))
         print("mixed gradient sweep", name, errs, flush=True)
         assert errs["h"] < 1e-6 and errs["rho"] < 1e-6

@@ -19,7 +19,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-TUNE_DEFAULTS = dict(DEFAULT_ITEMS, bucket=64, loop_blocks_scale=4, tail_leaves=2048, cull=1, units=1)
+TUNE_DEFAULTS = dict(DEFAULT_ITEMS, bucket=64, loop_blocks_scale=4, tail_leaves=2048, cull=1, units=1, tile_depth=64)
 
 
 @pytest.fixture(autouse=True)
@@ -55,7 +55,7 @@ def test_bucket_tree_and_solve(name, bucket):
 
 CANDIDATES = {"loop": {0, 1}, "bucket": {32, 64, 128}, "mapping": {0, 1, 2}, "sparse": {4, 8, 16, 32, 64},
               "tail_leaves": {512, 2048, 8192, 32768}, "loop_blocks_scale": {2, 4, 8}, "graph_passes": {1, 2, 4, 8},
-              "lpt": {0, 1}, "cull": {0, 1}, "units": {0, 1}}
+              "lpt": {0, 1}, "cull": {0, 1}, "units": {0, 1}, "tile_depth": {64, 128}}
 
 
 def test_autotune_single_set():
