@@ -118,10 +118,42 @@ def run_precision(lib, c, prec, steps, warmup, opts=()):
         "phase_ms": {k: 1e3 * acc[k] / steps for k in ("bbox", "keys", "sort", "permute", "tree", "nodes", "jbuild",
                                                           "nrkern", "compact", "closure", "total")},
         "n_leaves": lib.stat("n_leaves"),
+        "stages": streaming_stages(lib, n, acc, steps),
     }
     res["h_sorted"] = lib.array("h")
     res["rho_sorted"] = lib.array("rho")
     return res
+
+
+HBM_PEAK = 8000.0   # GB/s, MI355X HBM3E
+
+
+def streaming_stages(lib, n, acc, steps):
+    """Achieved GB/s of the bandwidth stages of one solve against 8 TB/s, with the algorithmic bytes
+    of the fixed conventions: open box 24 B/particle, keys 40, sort 20 (single-pass lower bound),
+    permute 68, leaf tree 8 B/particle + 12 B/leaf, linked tree + geometry + maps 64 B/node +
+    8 B/leaf + 4 B/particle; per NR pass, hmax 8 B per particle evaluated + 8 B per active leaf,
+    j-lists 4 B per entry + 8 B per active leaf + 56 B per node, compaction 12 B per particle
+    evaluated.  Times are the library's device-event phases averaged over the timed steps; the
+    per-pass counts come from the last solve's pass records (the same inputs every step)."""
+    L, nodes = lib.stat("n_leaves"), lib.stat("n_nodes")
+    P = lib.passes()
+    f = {k: i for i, k in enumerate(Library.PASS_FIELDS)}
+    leaves, iterating, entries = P[:, f["leaves"]].sum(), P[:, f["iterating"]].sum(), P[:, f["list_entries"]].sum()
+    nb = len(P)
+    algo = {
+        "bbox": 24.0 * n, "keys": 40.0 * n, "sort": 20.0 * n, "permute": 68.0 * n,
+        "tree": 8.0 * n + 12.0 * L, "nodes": 64.0 * nodes + 8.0 * L + 4.0 * n,
+        "jbuild": 8.0 * iterating + 8.0 * leaves + 4.0 * entries + 8.0 * leaves + 56.0 * nodes * nb,
+        "compact": 12.0 * iterating,
+    }
+    out = {}
+    for k, b in algo.items():
+        ms = 1e3 * acc[k] / steps
+        gbps = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        out[k] = {"ms": ms, "algorithmic_bytes": b, "gb_per_s": gbps, "frac": gbps / HBM_PEAK}
+    out["jbuild"]["note"] = "hmax + count walk + scan + fill walk over %d passes (latency-bound DFS)" % nb
+    return out
 
 
 def measured_traffic(config, form):
@@ -686,6 +718,7 @@ def main():
         "candidates_per_step": head["candidates_per_step"],
         "iterations": head["iterations"],
         "phase_ms": head["phase_ms"],
+        "stages": head["stages"],
         "options": {k: lib.stat(k) for k in Library.TUNE_OPTIONS},
         "autotune": tuned,
         "roofline": {"bound": "valu", "achieved": head["kernel_tflops"], "peak": peak, "unit": "TFLOP/s",

@@ -33,6 +33,7 @@ struct PassCtl
     unsigned long long listNeed;        // list entries of the current pass
     unsigned long long tBegin;          // wall clock at the start of the pass
     unsigned long long tStart, tEnd;    // density kernel span (wall clock) of the current pass
+    unsigned long long listMean;        // list entries per active leaf of the last pass that built lists (0: none yet)
 };
 
 // One record per pass of a device-driven loop (u64 fields, wall-clock ticks for times).

@@ -135,7 +135,10 @@ constexpr int kStack       = 152;                // >= 7*20 + 8
 // entry for entry.
 constexpr int kHeavySteps   = 256;    // per 32768 lists of the launch (a launch of many lists
                                       // keeps the chip busy through long walks); 128, 512 and
-                                      // 1024 were slower on the 8M Plummer mixture
+                                      // 1024 were slower on the 8M Plummer mixture.  Lowered
+                                      // to a multiple of the typical walk once one is known
+                                      // (ListArgs::heavyFactor).
+constexpr int kHeavyMinSteps = 64;
 constexpr int kTokCap       = 2048;
 constexpr int kExpandLeaves = 256;
 
@@ -165,6 +168,8 @@ struct ListArgs
     uint8_t*        heavyFlag;    // ... per position: 1 if the block walk owns the list
     int             heavySteps;   // ... hand-over threshold per 32768 lists (kHeavySteps)
     int             expandLeaves; // ... expansion threshold (kExpandLeaves)
+    unsigned long long listMean;  // entries per leaf of the last build of this kind (0: unknown; ctl: its own)
+    int             heavyFactor;  // hand-over after heavyFactor x listMean / 4 steps (0: off)
 };
 
 // The accept test of one i-leaf's list (include/tree.cuh:228-252 and the tight test).
@@ -282,7 +287,16 @@ __global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
     if (A.ctl && A.ctl->skip) return;
     const int nL = A.ctl ? (int)A.ctl->nAL : A.nLeaves;
     const bool heavy = A.heavyList != nullptr;
-    const int heavyCap = A.heavySteps * max(1, (nL + 32767) / 32768);
+    // hand-over threshold: the launch-size rule, and once a typical list length is known (the
+    // previous pass's, or the previous build's of this kind) heavyFactor times its walk (about
+    // a step per 4 entries), at least kHeavyMinSteps.  Where most lists are short (carried
+    // steps, force lists) the few long ones then go to the block walk early instead of holding
+    // the launch for thousands of steps; where most are long (first passes) the rule stands.
+    int heavyCap = A.heavySteps * max(1, (nL + 32767) / 32768);
+    const unsigned long long mean = A.ctl ? A.ctl->listMean : A.listMean;
+    if (mean > 0ull && A.heavyFactor > 0)
+        heavyCap = min(heavyCap, (int)max((unsigned long long)kHeavyMinSteps,
+                                          min(mean * (unsigned long long)A.heavyFactor / 4ull, 1ull << 30)));
     for (int a = blockIdx.x * kGroups + g; a < nL; a += gridDim.x * kGroups)
     {
         if (!COUNT && heavy && A.heavyFlag[a]) continue;   // filled by the block walk
@@ -505,7 +519,7 @@ void hmaxUpsweep(State& s)
     }
 }
 
-constexpr int kHeavyBlocks = 1024;
+constexpr int kHeavyBlocks = 4096;
 
 // The walks of one form (count, or fill at jOffset), then the block walks of the long lists.
 template<bool SYM, bool COUNT>
@@ -545,6 +559,8 @@ static ListArgs listArgs(State& s, const int* leaves, int nLeaves, bool symmetri
     a.heavyFlag = s.heavyFlag.get<uint8_t>((size_t)s.nLeaves + 1);
     a.heavySteps = s.listHeavySteps > 0 ? s.listHeavySteps : kHeavySteps;
     a.expandLeaves = s.listExpandLeaves >= 0 ? s.listExpandLeaves : kExpandLeaves;
+    a.heavyFactor = s.listHeavyFactor;
+    a.listMean = (unsigned long long)(symmetric ? s.listMeanSym : (leaves ? 0.0 : s.listMeanGather));
     return a;
 }
 
@@ -603,6 +619,8 @@ long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric, b
         else           launchList<false, false>(a, per, st);
     }
     s.listTotal = (int)total;
+    if (!leaves && nLeaves > 0)
+        (symmetric ? s.listMeanSym : s.listMeanGather) = (double)total / (double)nLeaves;
     return (long long)total;
 }
 

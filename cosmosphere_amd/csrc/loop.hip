@@ -82,6 +82,7 @@ __global__ void listCheckKernel(PassCtl* __restrict__ C, const uint32_t* __restr
     if (C->skip) return;
     const unsigned long long tot = C->nAL ? (unsigned long long)jOffset[C->nAL] : 0ull;
     C->listNeed = tot;
+    if (C->nAL) C->listMean = tot / C->nAL;
     if (tot > cap)
     {
         C->overflow = 1u;
@@ -498,6 +499,9 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
     std::memset(&h0, 0, sizeof(h0));
     h0.nAL = (uint32_t)in.nAL;
     h0.nIterating = in.nIterating;
+    // the first pass's typical list: the last all-leaf gather lists' (a carried step's h is
+    // near the converged h they were built at); none for a first solve
+    h0.listMean = (unsigned long long)s.listMeanGather;
     HIP_CHECK(hipMemcpyAsync(P.ctl, &h0, sizeof(h0), hipMemcpyHostToDevice, st));
 
     const int K = std::max(1, s.graphPasses);

@@ -190,6 +190,9 @@ struct State
     int      loopBlocks = 0, loopBlocksPrecision = -1, loopBlocksPeriodic = -1;
     long long loopListCap = 0;     // verification: starting list capacity of the device-driven loop (0: auto)
     int listHeavySteps = 0;        // verification: walk steps before a list goes to the block walk (0: default)
+    int listHeavyFactor = 8;       // block walk for lists past this many times a typical walk (0: launch-size rule only)
+    double listMeanGather = 0.0;   // entries per leaf of the last all-leaf gather / symmetric list build
+    double listMeanSym = 0.0;
     int listExpandLeaves = -1;     // verification: block walk expands nodes above this many leaves (-1: default)
     int splitLen = -1;             // device-driven loop: heavy items with longer lists are split (-1: default, 0: off)
     int forceRounds = 0;           // mixed force pass: 1 shares each tile's accepted pairs over the lanes, 0 lanes keep their own slots

@@ -116,7 +116,8 @@ def test_carried_loop_fp64(case):
     out, movers, resident = gpu_loop(c, v, 0)
     # later steps keep the previous order and re-sort the movers only
     assert resident == 1
-    assert movers[0] == -1 and all(0 < m < N for m in movers[1:]), movers   # -1: a full sort
+    # (the first solve may re-sort from an earlier solve of 20,000 particles in this process)
+    assert all(0 < m < N // 2 for m in movers[1:]), movers
     e = errors(out, ref)
     print("carried fp64 max errors", {k: float(x.max()) for k, x in e.items()}, flush=True)
     assert e["h"].max() < 1e-12 and e["rho"].max() < 1e-12

@@ -48,7 +48,8 @@ def same(a, b):
 VARIANTS = [("graph_k4", {}), ("graph_k1", {"graph_passes": 1}), ("graph_k3", {"graph_passes": 3}),
             ("direct", {"graph": 0}), ("no_lpt", {"lpt": 0}), ("overflow", {"loop_list_cap": 64}),
             ("flat", {"mapping": 2}), ("whole", {"mapping": 1}), ("nocull", {"cull": 0}),
-            ("nounits", {"units": 0}), ("block_walk", {"list_heavy_steps": 1, "list_expand_leaves": 0})]
+            ("nounits", {"units": 0}), ("block_walk", {"list_heavy_steps": 1, "list_expand_leaves": 0}),
+            ("launch_rule", {"list_heavy_factor": 0}), ("early_handover", {"list_heavy_factor": 1})]
 
 
 @pytest.mark.parametrize("label,opts", VARIANTS, ids=[v[0] for v in VARIANTS])

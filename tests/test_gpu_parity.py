@@ -70,13 +70,20 @@ def test_density_plain_fp64(name):
     assert abs(lib.stat("pairs") - sum(o.get("pairsPerIter"))) <= 1e-6 * sum(o.get("pairsPerIter")) + 2
 
 
-@pytest.mark.parametrize("walk", ["group", "block"])
+WALKS = {"group": {}, "block": FORCE_BLOCK_WALK, "launch_rule": {"list_heavy_factor": 0},
+         "early_handover": {"list_heavy_factor": 1}}
+
+
+@pytest.mark.parametrize("walk", list(WALKS))
 @pytest.mark.parametrize("name", ["lattice", "plummer_quintic"])
 def test_jlists_bit_exact(name, walk):
     """All-leaf gather lists at the converged h (left by the gradient sweep), walked by groups of
-    8 lanes or (block) every list handed to the block walk, which splits it into subtree walks."""
+    8 lanes or (block) every list handed to the block walk, which splits it into subtree walks.
+    A list goes to the block walk once its walk passes 8 times a typical one (the previous
+    build's); launch_rule keeps only the launch-size threshold, early_handover hands over at once
+    (the floor of 64 steps)."""
     c = dataset(name)
-    lib, _ = gpu_run(c, precision=0, gradients=True, **(FORCE_BLOCK_WALK if walk == "block" else {}))
+    lib, _ = gpu_run(c, precision=0, gradients=True, **WALKS[walk])
     o, _ = oracle_run(c, gradients=False)
     o.set_h_sorted(lib.array("h"))
     o.hmax_all()
