@@ -245,7 +245,8 @@ def cpu_baseline(c, o, t_tree, gpu_h_fp64, gpu_h_mixed, seconds):
 def parity_full(lib, c, o, sample):
     """Full-size parity against the FP64 oracle (the GPU's plain FP64 path, the oracle's tree):
     keys, order, leaf tree and layout bit for bit; the all-leaf gather lists at the GPU's
-    converged h (one FP64 solve with the gradient sweep, which rebuilds them) bit for bit; h and
+    converged h (one FP64 solve with the gradient sweep over gather lists, grad_sym_lists=0)
+    bit for bit; h and
     rho of the sampled leaves' particles (the cpu_baseline sample), with the particles whose
     Newton iteration stopped one step apart counted separately (|dh/h| above 1e-12: summation
     order alone moves h by rounding, about 1e-15; a |dh/h| < tolh decision that rounding tipped
@@ -254,7 +255,12 @@ def parity_full(lib, c, o, sample):
     lib.set_option("precision", 0)
     lib.set_option("closure", -1)
     lib.restore_inputs()
-    lib.solve_dens_h(c["n"], c["pmass"], 0, True, None, 1e-4, c["hfact"])
+    sym = lib.stat("grad_sym_lists")
+    lib.set_option("grad_sym_lists", 0)
+    try:
+        lib.solve_dens_h(c["n"], c["pmass"], 0, True, None, 1e-4, c["hfact"])
+    finally:
+        lib.set_option("grad_sym_lists", sym)
     out = {}
     cmp = {"keys": "keys", "order": "order", "leafLevel": "leafLevel", "layout": "layout"}
     for g, r in cmp.items():

@@ -305,6 +305,7 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "loop_list_cap") s.loopListCap = (long long)value;
     else if (nm == "list_heavy_steps") s.listHeavySteps = (int)value;
     else if (nm == "list_heavy_factor") s.listHeavyFactor = std::max(0, (int)value);
+    else if (nm == "grad_sym_lists") s.gradSymLists = (int)value;
     else if (nm == "split_len") s.splitLen = (int)value;
     else if (nm == "sweep_split") s.sweepSplit = std::max(0, (int)value);
     else if (nm == "list_expand_leaves") s.listExpandLeaves = (int)value;
@@ -383,6 +384,7 @@ extern "C" double cosmo_ext_get_stat(const char* name)
     if (nm == "resident") return s.resident;
     if (nm == "force_rounds") return s.forceRounds;
     if (nm == "list_heavy_factor") return s.listHeavyFactor;
+    if (nm == "grad_sym_lists") return s.gradSymLists;
     if (nm == "tile_depth") return s.tileDepth;
     if (nm == "resort_movers") return t.resortMovers;
     if (nm == "upload_s") return s.uploadTotal;

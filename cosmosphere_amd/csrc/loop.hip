@@ -499,9 +499,10 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
     std::memset(&h0, 0, sizeof(h0));
     h0.nAL = (uint32_t)in.nAL;
     h0.nIterating = in.nIterating;
-    // the first pass's typical list: the last all-leaf gather lists' (a carried step's h is
-    // near the converged h they were built at); none for a first solve
-    h0.listMean = (unsigned long long)s.listMeanGather;
+    // the first pass's typical list: the last all-leaf lists' (a carried step's h is near the
+    // converged h they were built at; the symmetric lists of the gradient sweep, a superset,
+    // when no gather lists were built); none for a first solve
+    h0.listMean = (unsigned long long)(s.listMeanGather > 0.0 ? s.listMeanGather : s.listMeanSym);
     HIP_CHECK(hipMemcpyAsync(P.ctl, &h0, sizeof(h0), hipMemcpyHostToDevice, st));
 
     const int K = std::max(1, s.graphPasses);

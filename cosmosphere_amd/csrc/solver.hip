@@ -335,12 +335,17 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         s.listSymmetric = 0;
     }
 
+    // the gradient sweep over the symmetric (force) lists: a superset of the gather lists in the
+    // same order (a node within reach of the i-leaf's hmax is within reach of the larger of the
+    // two hmax), so the sweep's pairs are the same; the force pass then reuses the lists
+    const bool symLists = withGradients && s.gradSymLists;
     if (withGradients)
     {
         Ev g0, g1, g2;
         g0.rec(st);
         hmaxLeaves(s, nullptr, L);
-        buildLists(s, nullptr, L, false);
+        if (symLists) hmaxUpsweep(s);
+        buildLists(s, nullptr, L, symLists);
         g1.rec(st);
         GradArgs g{};
         g.nLeaves = na > 0 ? L : 0;
@@ -368,13 +373,14 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         t.gjbuild = secs(g0, g1);
         t.gradkern = secs(g1, g2);
         s.listLeaves = L;
-        s.listSymmetric = 0;
+        s.listSymmetric = symLists ? 1 : 0;
     }
     e1.rec(st);
     HIP_CHECK(hipEventSynchronize(e1.e));
     t.total = secs(e0, e1);
     s.t = t;
     ++s.token;
+    if (symLists) s.jlistToken = s.token;   // the force pass of this solve reuses them
     return t;
 }
 

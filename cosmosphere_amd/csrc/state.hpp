@@ -190,6 +190,7 @@ struct State
     int      loopBlocks = 0, loopBlocksPrecision = -1, loopBlocksPeriodic = -1;
     long long loopListCap = 0;     // verification: starting list capacity of the device-driven loop (0: auto)
     int listHeavySteps = 0;        // verification: walk steps before a list goes to the block walk (0: default)
+    int gradSymLists = 1;          // gradient sweep over the symmetric lists, which the force pass reuses
     int listHeavyFactor = 8;       // block walk for lists past this many times a typical walk (0: launch-size rule only)
     double listMeanGather = 0.0;   // entries per leaf of the last all-leaf gather / symmetric list build
     double listMeanSym = 0.0;

@@ -83,8 +83,9 @@ def _default_paths():
 @pytest.mark.parametrize("paths", ["default", "forced", "mixed"])
 def test_periodic_box(paths, _default_paths):
     """Periodic solve with gradients against the oracle: FP64 (default and forced block walks and
-    split items) within 1e-10 (h, rho) and 1e-7 of the largest |div v|, lists bit-exact; the mixed
-    form within 1e-6 (h, rho) and 2e-5 of the largest |div v|."""
+    split items) within 1e-10 (h, rho) and 1e-12 of the largest |div v|, lists bit-exact; the
+    mixed form within 1e-6 (h, rho), 2e-6 of the largest |div v| and 1e-3 of each particle's own
+    (floor 1e-3 of the largest)."""
     c, box = periodic_lattice()
     n = c["n"]
     rng = np.random.default_rng(6)
@@ -109,7 +110,12 @@ def test_periodic_box(paths, _default_paths):
         assert rel_err(out[name], ref) < tol, name
     ref = np.empty(n)
     ref[order] = o.get("divv")
-    assert np.max(np.abs(out["divv"] - ref)) < (2e-5 if paths == "mixed" else 1e-7) * np.abs(ref).max()
+    err = np.max(np.abs(out["divv"] - ref)) / np.abs(ref).max()
+    err_pp = rel_err(out["divv"], ref, floor=np.abs(ref).max() * 1e-3)
+    print("periodic divv error", paths, err, err_pp, flush=True)
+    # measured: FP64 1.6e-15 of the largest (3e-13 per particle); mixed 3.6e-7 (1.3e-4)
+    assert err < (2e-6 if paths == "mixed" else 1e-12)
+    assert err_pp < (1e-3 if paths == "mixed" else 1e-10)
     if paths != "mixed":
         np.testing.assert_array_equal(lib.array("jList"), o.get("jList"))
 

@@ -74,20 +74,22 @@ WALKS = {"group": {}, "block": FORCE_BLOCK_WALK, "launch_rule": {"list_heavy_fac
          "early_handover": {"list_heavy_factor": 1}}
 
 
+@pytest.mark.parametrize("kind", ["gather", "symmetric"])
 @pytest.mark.parametrize("walk", list(WALKS))
 @pytest.mark.parametrize("name", ["lattice", "plummer_quintic"])
-def test_jlists_bit_exact(name, walk):
-    """All-leaf gather lists at the converged h (left by the gradient sweep), walked by groups of
-    8 lanes or (block) every list handed to the block walk, which splits it into subtree walks.
-    A list goes to the block walk once its walk passes 8 times a typical one (the previous
-    build's); launch_rule keeps only the launch-size threshold, early_handover hands over at once
-    (the floor of 64 steps)."""
+def test_jlists_bit_exact(name, walk, kind):
+    """All-leaf lists at the converged h, left by the gradient sweep: the symmetric (force) lists
+    by default, the gather lists with grad_sym_lists=0.  Walked by groups of 8 lanes or (block)
+    every list handed to the block walk, which splits it into subtree walks.  A list goes to the
+    block walk once its walk passes 8 times a typical one (the previous build's); launch_rule
+    keeps only the launch-size threshold, early_handover hands over at once (the floor of 64
+    steps)."""
     c = dataset(name)
-    lib, _ = gpu_run(c, precision=0, gradients=True, **WALKS[walk])
+    lib, _ = gpu_run(c, precision=0, gradients=True, grad_sym_lists=int(kind == "symmetric"), **WALKS[walk])
     o, _ = oracle_run(c, gradients=False)
     o.set_h_sorted(lib.array("h"))
     o.hmax_all()
-    o.build_lists(symmetric=False)
+    o.build_lists(symmetric=(kind == "symmetric"))
     np.testing.assert_array_equal(lib.array("hmaxLeaf"), o.get("hmaxLeaf"))
     np.testing.assert_array_equal(lib.array("jCount"), o.get("jCount"))
     np.testing.assert_array_equal(lib.array("jOffset"), o.get("jOffset"))
@@ -99,8 +101,9 @@ def test_jlists_bit_exact(name, walk):
                                                   ("plummer_quintic", 512, 1), ("plummer_quintic", 4, 1)])
 def test_gradient_sweep(name, sweep, precision):
     """Gradient sweep against the FP64 oracle: plain FP64 within 1e-10 (h, rho) and 1e-7 (div v,
-    d(div v)/dt); the mixed form (FP32 pair terms, FP64 rho sums) within 1e-6 (h, rho) and 2e-5 of
-    the largest |div v|, |d(div v)/dt| (xi within 2e-5)."""
+    d(div v)/dt); the mixed form (FP32 pair terms and gradient sums, FP64 rho sums) within 1e-6
+    (h, rho), 2e-6 of the largest |div v|, |d(div v)/dt| and 1e-3 of each particle's own value
+    (with a floor of 1e-3 of the largest), xi within 1e-5."""
     c = dataset(name)
     n = c["n"]
     idx = np.arange(n)
@@ -145,8 +148,11 @@ def test_gradient_sweep(name, sweep, precision):
             errs[f + "_pp"] = rel_err(out[f], ref, floor=np.abs(ref).max() * 1e-3)
         errs["xi"] = np.max(np.abs(out["xi"] - host("xi")))
         print("mixed gradient sweep", name, errs, flush=True)
+        # measured: h, rho below 1.4e-7; div v, d(div v)/dt below 5.6e-7 of their largest
+        # magnitude and below 1.1e-4 of each particle's own (floor 1e-3 of the largest); xi 2.3e-6
         assert errs["h"] < 1e-6 and errs["rho"] < 1e-6
-        assert errs["divv"] < 2e-5 and errs["ddivvdt"] < 2e-5 and errs["xi"] < 2e-5
+        assert errs["divv"] < 2e-6 and errs["ddivvdt"] < 2e-6 and errs["xi"] < 1e-5
+        assert errs["divv_pp"] < 1e-3 and errs["ddivvdt_pp"] < 1e-3
 
 
 @pytest.mark.parametrize("name", ["lattice", "C1small", "plummer_cubic", "plummer_quintic"])
