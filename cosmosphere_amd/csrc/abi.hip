@@ -73,6 +73,8 @@ double secondsSince(Clock::time_point t0) { return std::chrono::duration<double>
 
 // Input slots land in the resident order when the last solve was this set; everything else
 // in the caller's order (it is then sorted by the solve).
+constexpr int kSlotHsml = 2;
+
 void upload(int slot, const double* host, int n, bool sortedCall)
 {
     if (n <= 0) return;
@@ -86,6 +88,8 @@ void upload(int slot, const double* host, int n, bool sortedCall)
     // a sorted upload is read as caller order by a later solve that finds it not uploaded
     // again (the reference's convention); a resident one as the current order
     for (int c = 0; d.firstInput >= 0 && c < d.ncomp; ++c) s.inTag[d.firstInput + c] = resident ? kTagAligned : kTagCaller;
+    // new smoothing lengths: a force pass builds its lists again (their reach is hmax)
+    if (slot == kSlotHsml) s.jlistToken = 0;
     s.uploadSeconds += secondsSince(t0);
 }
 

@@ -124,6 +124,29 @@ def test_force_repeat_reuses_lists():
     np.testing.assert_array_equal(lib.download(HSML, n)[0], out["h"])
 
 
+def test_force_after_hsml_upload_rebuilds_lists():
+    """New smoothing lengths uploaded between the solve and the force pass: the force pass builds
+    its symmetric lists again from them (bit-exact against the oracle's at the new h) instead of
+    reusing the lists of the solve's gradient sweep."""
+    c, v, u = setup(n=8000, seed=4)
+    n = c["n"]
+    lib = Library(c["kernel"])
+    lib.set_option("precision", 0)
+    out = lib.solve(c["x"], c["y"], c["z"], c["h"], c["pmass"], hfact=c["hfact"], v=v)
+    h2 = out["h"] * (1.0 + 0.2 * workloads.uniform(4, 50, np.arange(n)))
+    lib.upload(HSML, h2)
+    lib.upload_sorted(THERMO, thermo(c, h2, u))
+    lib.force(n, c["pmass"])
+    o = O.Oracle(quintic=c["quintic"])
+    o.set_particles(c["x"], c["y"], c["z"], c["h"], v=v, a=np.zeros((3, n)))
+    o.build()
+    order = o.get("order")
+    o.set_h_sorted(h2[order])
+    o.hmax_all()
+    o.build_lists(symmetric=True)
+    np.testing.assert_array_equal(lib.array("jList"), o.get("jList"))
+
+
 @pytest.mark.parametrize("variant", [dict(), dict(disc=1), dict(shock=0, quintic=False), dict(sweep=4),
                                      dict(rounds=1), dict(rounds=1, sweep=4)])
 def test_force_mixed_vs_oracle(variant):
