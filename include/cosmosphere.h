@@ -119,7 +119,15 @@ int cosmo_ext_solve_dens_h(int n, double pmass, int mode, int withGradients, int
  * more leaves than this are expanded into separate walks, default 256), "split_len" (device-driven
  * loop: a heavy work item whose list has more entries is split over several wavefronts, default
  * 2048; 0: never), "sweep_split" (gradient sweep and force pass: the particles of a leaf whose
- * list has more entries go one per wavefront, default 512; 0: never).
+ * list has more entries go one per wavefront, default 512; 0: never), "list_heavy_factor" (a
+ * walk also goes to the block walk past this many times the previous build's typical walk,
+ * default 8; 0: the launch-size rule only), "tile_depth" (device-driven loop: j-slots per LDS
+ * tile of whole-leaf items, 64 or 128, the default), "grad_sym_lists" (1, the default: the
+ * gradient sweep runs over the symmetric lists, which the force pass of the same solve reuses;
+ * 0: over gather lists), "force_rounds" (mixed force pass: 1 spreads each tile's accepted pairs
+ * evenly over the lanes, 0 (the default) lanes keep their own particle), "resident" (1, the
+ * default: uploads after a single-set solve land in its Hilbert order and the next solve
+ * re-sorts incrementally), "xfer_threads" (host threads of the transfer pool, default 8).
  * The environment variable COSMO_PRECISION=mixed|fp64 sets the precision for
  * densityiterate_gpu_c. */
 void cosmo_ext_set_option(const char* name, double value);
@@ -151,8 +159,8 @@ int cosmo_batch_solve(const double* pmass, int periodic, const double* boxes, do
  * options the results do not depend on -- NR loop driver ("loop"), leaf bucket size ("bucket"),
  * wave-per-leaf vs flat item mapping ("mapping", "sparse", "tail_leaves"), persistent grid size
  * ("loop_blocks_scale"), passes per graph replay ("graph_passes"), heavy items first ("lpt"),
- * per-particle culling of list entries ("cull"), lanes over work units ("units")
- * -- one option at a time, and leaves the fastest in force.  The uploaded inputs are put back
+ * per-particle culling of list entries ("cull"), lanes over work units ("units"), j-tile depth
+ * ("tile_depth") -- one option at a time, and leaves the fastest in force.  The uploaded inputs are put back
  * afterwards (a following solve starts from them; download nothing before it).  budgetSeconds
  * bounds the search.  A workload with the same signature (particle count, set sizes, kernel,
  * precision, tolerances) reuses the earlier choice (also across processes through the file named
