@@ -17,7 +17,7 @@ import pytest
 
 from cosmosphere_amd import Library, workloads
 from cosmosphere_amd.api import ACCEL, DENS_OUT, FORCE_OUT, HSML, POS, THERMO, VEL
-from gpu_common import DEFAULT_ITEMS, dataset
+from gpu_common import DEFAULT_ITEMS, dataset, periodic_lattice
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -71,6 +71,52 @@ def test_incremental_resort_equals_full_sort(name, tree_only):
                 h = full["h_caller"]
     finally:
         lib.set_option("tree_only", 0)
+        lib.set_option("resident", 1)
+
+
+def test_incremental_resort_periodic_wrap():
+    """A periodic box: drifting particles leave through one face and come back through the
+    opposite one, so their keys jump across the whole curve.  Each step is solved from the
+    resident order and again from the caller's order with the resident path off, as above."""
+    c, box = periodic_lattice()
+    n = c["n"]
+    lib = Library(c["kernel"])
+    for k, val in DEFAULT_ITEMS.items():
+        lib.set_option(k, val)
+    lib.set_option("precision", 0)
+    lib.set_option("max_iter", 100)
+    lib.set_option("tree_only", 0)
+    lib.arrays_init(n)
+    rng = np.random.default_rng(23)
+    v = rng.normal(size=(3, n))
+    lo, span = box[0::2], box[1::2] - box[0::2]
+    h = c["h"].copy()
+
+    def solve(x, y, z, resident):
+        lib.set_option("resident", int(resident))
+        lib.upload(POS, np.stack([x, y, z]))
+        lib.upload(HSML, h)
+        lib.upload(VEL, v)
+        lib.upload(ACCEL, np.zeros((3, n)))
+        lib.solve_dens_h(n, c["pmass"], with_gradients=True, periodic_box=box, tolh=1e-4, hfact=1.2)
+        out = {k: lib.array(k) for k in COMPARED}
+        out["movers"] = lib.stat("resort_movers")
+        out["h_caller"] = lib.download(HSML, n)[0]
+        return out
+
+    try:
+        for step, dt in enumerate((0.0, 3e-3, 3e-2)):
+            x, y, z = ((np.stack([c["x"], c["y"], c["z"]]) + v * dt - lo[:, None]) % span[:, None]) + lo[:, None]
+            res = solve(x, y, z, True)
+            full = solve(x, y, z, False)
+            assert full["movers"] == -1
+            if step > 0:
+                assert 0 < res["movers"] <= n, res["movers"]
+            for k in full:
+                if k != "movers":
+                    np.testing.assert_array_equal(res[k], full[k], err_msg="periodic step %d %s" % (step, k))
+            h = full["h_caller"]
+    finally:
         lib.set_option("resident", 1)
 
 
