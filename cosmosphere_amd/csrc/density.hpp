@@ -220,7 +220,7 @@ void launchForcePrep(const double* h, const double* gradh, const double* pro2, d
                      double* rhoh, double* rho1, double* grkfac, double* pres, double* auterm, double* divfac, int n,
                      double pmass, double alphau, double hfact, hipStream_t st);
 // rounds: the mixed form with each tile's accepted pairs shared evenly over the lanes
-// (forceKernelMixedRounds), else lanes that keep their own slots (forceKernelMixed)
+// (forceKernelMixedRounds, the default), else lanes that keep their own slots (forceKernelMixed)
 void launchForce(const ForceArgs& a, bool periodic, bool disc, bool full, bool mixed, bool rounds, hipStream_t st);
 // FP32 copies of the force factors for the mixed form (ForceArgs::f32 layout), n particles.
 void launchForceF32(const ForceArgs& a, int n, float* dst, hipStream_t st);

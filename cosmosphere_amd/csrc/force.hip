@@ -565,21 +565,24 @@ __device__ inline int nthSetBit(uint64_t x, int k)
     return pos;
 }
 
-// Mixed form with shared slots (the default): the same lists, culling, tile and pair terms as
+// Mixed form with shared slots (force_rounds=1): the same lists, culling, tile and pair terms as
 // forceKernelMixed, with the accepted pairs of a tile spread evenly over the wavefront.  Lane p
 // owns particle p of the chunk (up to 64 at a time).  For each tile:
 //   1. the support test of each particle runs over the 64 slots at once (its position read as
-//      scalars), giving its mask of accepted slots;
+//      scalars), giving its mask of accepted slots, kept in the particle's lane;
 //   2. with a_p accepted slots for particle p, the smallest m with sum ceil(a_p / m) <= 64 gives
 //      each particle ceil(a_p / m) consecutive lanes, each taking up to m of its accepted slots;
 //   3. each lane pulls its particle's factors from the owner lane once, evaluates its m pairs
-//      into FP32 partials, and adds them to the particle's partials in LDS (atomic adds within
-//      the wavefront, contended only by the lanes sharing the particle);
+//      into FP32 partials (i- and j-side terms together), and adds them to the particle's
+//      partials in LDS (atomic adds within the wavefront, contended only by the lanes sharing
+//      the particle);
 //   4. the owners add the tile's partials into their FP64 sums.
 // A wavefront then runs the pair terms about total / 64 times per tile; with lanes that keep
-// their own slots it runs them as often as its busiest lane accepts a pair.
+// their own slots it runs them as often as its busiest lane accepts a pair.  To fit four
+// wavefronts per SIMD (128 VGPRs, under 40 KB of LDS per block) the chunk's positions are read
+// from their owner lanes, not from LDS, and the chunk's box lives in LDS.
 template<bool PER, bool DISC, bool FULL>
-__global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArgs A)
+__global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(4))) void forceKernelMixedRounds(ForceArgs A)
 {
     __shared__ int    segStart[kForceWaves][kWave];
     __shared__ int    segJ[kForceWaves][kWave];
@@ -588,10 +591,9 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArg
     enum { H, HS, HI, GK, RH, R1, P2, CS, AL, PR, AU, NS };
     __shared__ double td[kForceWaves][ND][kWave];
     __shared__ float  ts[kForceWaves][NS][kWave];
-    __shared__ double cp[kForceWaves][4][kWave];   // the chunk's particles: x, y, z, R h
     __shared__ int    jq[kForceWaves][2 * kWave];  // candidates that passed the particle test
-    __shared__ uint64_t acceptMask[kForceWaves][kWave];   // per particle: its accepted slots of the tile
-    enum { AIX, AIY, AIZ, AJX, AJY, AJZ, AF4, ADIV, AVM, NA };
+    __shared__ double cbox[kForceWaves][8];        // the chunk's box: centre, half-sizes, largest R h
+    enum { AX, AY, AZ, AF4, ADIV, AVM, NA };
     __shared__ float  pacc[kForceWaves][NA][kWave];   // the tile's partials per particle (AVM: the max's bits)
     const int w = threadIdx.x >> 6, lane = laneId();
     const size_t S = A.f32Stride;
@@ -611,38 +613,43 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArg
             const int c = __builtin_amdgcn_readfirstlane((int)min((uint32_t)kWave, p1 - q0));
             const bool own = lane < c;
             const uint32_t i = q0 + (uint32_t)(own ? lane : 0);
-            const double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+            const double xi = A.x[i], yi = A.y[i], zi = A.z[i], rhi = kRad * A.h[i];
             const double vxi = A.vx[i], vyi = A.vy[i], vzi = A.vz[i], eni = A.u[i];
             const float hif = (float)A.h[i];
             const float hsi = F(0, i), hinvi = F(1, i), gki = F(2, i), rhoi = F(3, i), rho1i = F(4, i);
             const float pro2i = F(5, i), vwi = F(6, i), ali = F(7, i), pri = F(8, i), aui = F(9, i);
-            double cbc[3], cbs[3], rChunk;   // the chunk's bounding box and largest reach R h
-            if (own)
             {
-                cp[w][0][lane] = xi; cp[w][1][lane] = yi; cp[w][2][lane] = zi;
-                cp[w][3][lane] = kRad * A.h[i];
+                double cbc[3], cbs[3], rChunk;
+                waveBox(own, xi, yi, zi, rhi, cbc, cbs, rChunk);
+                if (lane == 0)
+                {
+                    cbox[w][0] = cbc[0]; cbox[w][1] = cbc[1]; cbox[w][2] = cbc[2];
+                    cbox[w][3] = cbs[0]; cbox[w][4] = cbs[1]; cbox[w][5] = cbs[2];
+                    cbox[w][6] = rChunk;
+                }
             }
-            waveBox(own, xi, yi, zi, kRad * A.h[i], cbc, cbs, rChunk);
 #pragma unroll
             for (int f = 0; f < NA; ++f) pacc[w][f][lane] = 0.0f;
             waveLdsSync();
-            double ix = 0, iy = 0, iz = 0, jx = 0, jy = 0, jz = 0, f4s = 0, divs = 0;
+            double sx = 0, sy = 0, sz = 0, f4s = 0, divs = 0;
             float vmax = 0.0f;
             auto keep = [&](int jl) {
                 double hj = A.hmaxLeaf[jl];
                 const double rj = (isfinite(hj) && hj > 0.0) ? kRad * hj : 0.0;
                 for (int k = 0; k < c; ++k)
                 {
-                    const double r = fmax(cp[w][3][k], rj);
-                    if (pointLeafGap2<PER>(cp[w][0][k], cp[w][1][k], cp[w][2][k], A.lbox, A.lboxN, jl, box) <=
+                    const double r = fmax(laneD(rhi, k), rj);
+                    if (pointLeafGap2<PER>(laneD(xi, k), laneD(yi, k), laneD(zi, k), A.lbox, A.lboxN, jl, box) <=
                         r * r * kCullSlack)
                         return true;
                 }
                 return false;
             };
             auto keepJ = [&](int j) {
-                const double r = fmax(rChunk, kRad * A.h[j]);
-                return pointBoxGap2<PER>(A.x[j], A.y[j], A.z[j], cbc, cbs, box) <= r * r * kCullSlack;
+                const double cc[3] = {cbox[w][0], cbox[w][1], cbox[w][2]};
+                const double cs[3] = {cbox[w][3], cbox[w][4], cbox[w][5]};
+                const double r = fmax(cbox[w][6], kRad * A.h[j]);
+                return pointBoxGap2<PER>(A.x[j], A.y[j], A.z[j], cc, cs, box) <= r * r * kCullSlack;
             };
             auto load = [&](int l, int j) {
                 td[w][X][l] = A.x[j]; td[w][Y][l] = A.y[j]; td[w][Z][l] = A.z[j];
@@ -662,7 +669,8 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArg
             };
             auto consume = [&](int nt) {
                 nCand += (unsigned long long)nt * (unsigned long long)c;
-                // 1. each particle's accepted slots (slot `lane` against particle p)
+                // 1. each particle's accepted slots (slot `lane` against particle p), kept in lane p
+                uint64_t mine = 0;
                 {
                     const bool slot = lane < nt;
                     const int sl = slot ? lane : 0;
@@ -680,12 +688,10 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArg
                             __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, hif), p));
                         const float rmax2 = R2 * fmaxf(hp * hp, hj2) * (1.0f + 1e-6f);
                         const uint64_t m = ballot64(slot && (dr2 > 0.0f) && (dr2 < rmax2));
-                        if (lane == 0) acceptMask[w][p] = m;
+                        mine = lane == p ? m : mine;
                     }
                 }
-                waveLdsSync();
                 // 2. shares: m pairs per lane, ceil(a_p / m) lanes for particle p
-                const uint64_t mine = own ? acceptMask[w][lane] : 0;
                 const int ap = __popcll(mine);
                 const int total = waveReduceSum(ap);
                 if (total == 0) return;
@@ -707,12 +713,10 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArg
                 const int r = lane - __shfl(startP, pp, kWave);          // its share of the particle's pairs
                 const int apOf = __shfl(ap, pp, kWave);
                 const int cnt = p < 0 ? 0 : max(0, min(m, apOf - r * m));
+                const uint64_t mk = (uint64_t)(uint32_t)__shfl((int)(uint32_t)mine, pp, kWave) |
+                                    ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(mine >> 32), pp, kWave) << 32);
                 uint64_t bits = 0;
-                if (cnt > 0)
-                {
-                    const uint64_t mk = acceptMask[w][pp];
-                    bits = mk & ~((uint64_t(1) << nthSetBit(mk, r * m)) - 1);
-                }
+                if (cnt > 0) bits = mk & ~((uint64_t(1) << nthSetBit(mk, r * m)) - 1);
                 // 3. the particle's factors from its owner lane, then the lane's pairs
                 const double xp = pullD(xi, pp), yp = pullD(yi, pp), zp = pullD(zi, pp);
                 const double vxp = pullD(vxi, pp), vyp = pullD(vyi, pp), vzp = pullD(vzi, pp), enp = pullD(eni, pp);
@@ -721,7 +725,7 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArg
                 const float rhop = __shfl(rhoi, pp, kWave), rho1p = __shfl(rho1i, pp, kWave);
                 const float pro2p = __shfl(pro2i, pp, kWave), vwp = __shfl(vwi, pp, kWave);
                 const float alp = __shfl(ali, pp, kWave), prp = __shfl(pri, pp, kWave), aup = __shfl(aui, pp, kWave);
-                float tix = 0, tiy = 0, tiz = 0, tjx = 0, tjy = 0, tjz = 0, tf4 = 0, tdiv = 0, tvm = 0;
+                float tx = 0, ty = 0, tz = 0, tf4 = 0, tdiv = 0, tvm = 0;
                 for (int t = 0; t < m; ++t)
                 {
                     bool counted = false;
@@ -780,7 +784,7 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArg
                             tdiv += pm * dwk * projv;
                             const float gk = dwk * gkp;
                             const float gp = pm * (pro2p + qr2i) * gk;
-                            tix -= gp * rx; tiy -= gp * ry; tiz -= gp * rz;
+                            tx -= gp * rx; ty -= gp * ry; tz -= gp * rz;
                             const float pdvt = pm * pro2p * projv * gk;
                             const float diss = DISC ? -0.5f * pm * rho1p * alp * vwp * (hp * rij1) * projv * projv * gk
                                                     : pm * qr2i * projv * gk;
@@ -802,7 +806,7 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArg
                             kernWdW<float>(qj2, q, wk, dwk);
                             const float gk = dwk * ts[w][GK][k];
                             const float gp = pm * (pro2j + qr2j) * gk;
-                            tjx -= gp * rx; tjy -= gp * ry; tjz -= gp * rz;
+                            tx -= gp * rx; ty -= gp * ry; tz -= gp * rz;
                             tf4 += vsigu * denij * auj * gk;
                         }
                     }
@@ -811,17 +815,15 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArg
                 // 4. the lane's partials into its particle's
                 if (cnt > 0)
                 {
-                    atomicAdd(&pacc[w][AIX][pp], tix); atomicAdd(&pacc[w][AIY][pp], tiy);
-                    atomicAdd(&pacc[w][AIZ][pp], tiz); atomicAdd(&pacc[w][AJX][pp], tjx);
-                    atomicAdd(&pacc[w][AJY][pp], tjy); atomicAdd(&pacc[w][AJZ][pp], tjz);
-                    atomicAdd(&pacc[w][AF4][pp], tf4); atomicAdd(&pacc[w][ADIV][pp], tdiv);
+                    atomicAdd(&pacc[w][AX][pp], tx); atomicAdd(&pacc[w][AY][pp], ty);
+                    atomicAdd(&pacc[w][AZ][pp], tz); atomicAdd(&pacc[w][AF4][pp], tf4);
+                    atomicAdd(&pacc[w][ADIV][pp], tdiv);
                     atomicMax(reinterpret_cast<unsigned*>(&pacc[w][AVM][pp]), __float_as_uint(tvm) & 0x7FFFFFFFu);
                 }
                 waveLdsSync();
                 if (own)
                 {
-                    ix += (double)pacc[w][AIX][lane]; iy += (double)pacc[w][AIY][lane]; iz += (double)pacc[w][AIZ][lane];
-                    jx += (double)pacc[w][AJX][lane]; jy += (double)pacc[w][AJY][lane]; jz += (double)pacc[w][AJZ][lane];
+                    sx += (double)pacc[w][AX][lane]; sy += (double)pacc[w][AY][lane]; sz += (double)pacc[w][AZ][lane];
                     f4s += (double)pacc[w][AF4][lane]; divs += (double)pacc[w][ADIV][lane];
                     vmax = fmaxf(vmax, pacc[w][AVM][lane]);
                 }
@@ -833,10 +835,11 @@ __global__ __launch_bounds__(kForceThreads) void forceKernelMixedRounds(ForceArg
                                consume);
             if (own)
             {
-                A.fx[i] = ix + jx; A.fy[i] = iy + jy; A.fz[i] = iz + jz;
+                A.fx[i] = sx; A.fy[i] = sy; A.fz[i] = sz;
                 A.f4[i] = f4s; A.vsig[i] = (double)vmax;
                 A.divv[i] = -divs * A.divfac[i];
             }
+            waveLdsSync();   // cbox is rewritten for the next chunk
         }
     }
     if (laneId() == 0 && (nAcc | nCand))

@@ -196,7 +196,7 @@ struct State
     double listMeanSym = 0.0;
     int listExpandLeaves = -1;     // verification: block walk expands nodes above this many leaves (-1: default)
     int splitLen = -1;             // device-driven loop: heavy items with longer lists are split (-1: default, 0: off)
-    int forceRounds = 0;           // mixed force pass: 1 shares each tile's accepted pairs over the lanes, 0 lanes keep their own slots
+    int forceRounds = 1;           // mixed force pass: 1 shares each tile's accepted pairs over the lanes, 0 lanes keep their own slots
     int sweepSplit = 512;          // gradient / force sweeps: leaves with more list entries go one particle per wavefront (0: off)
     std::vector<PassRecord> passLog;   // one record per NR pass of the last solve
     int      closureOn = -1;   // -1: on for mixed, off for plain

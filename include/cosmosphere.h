@@ -124,8 +124,8 @@ int cosmo_ext_solve_dens_h(int n, double pmass, int mode, int withGradients, int
  * default 8; 0: the launch-size rule only), "tile_depth" (device-driven loop: j-slots per LDS
  * tile of whole-leaf items, 64 or 128, the default), "grad_sym_lists" (1, the default: the
  * gradient sweep runs over the symmetric lists, which the force pass of the same solve reuses;
- * 0: over gather lists), "force_rounds" (mixed force pass: 1 spreads each tile's accepted pairs
- * evenly over the lanes, 0 (the default) lanes keep their own particle), "resident" (1, the
+ * 0: over gather lists), "force_rounds" (mixed force pass: 1, the default, spreads each tile's
+ * accepted pairs evenly over the lanes; 0: lanes keep their own particle), "resident" (1, the
  * default: uploads after a single-set solve land in its Hilbert order and the next solve
  * re-sorts incrementally), "xfer_threads" (host threads of the transfer pool, default 8).
  * The environment variable COSMO_PRECISION=mixed|fp64 sets the precision for
