@@ -600,7 +600,9 @@ def bench_force(args, n_override=None):
                 "roofline": {"bound": "valu", "achieved": res[head]["force_kernel_tflops"], "peak": peak,
                              "unit": "TFLOP/s", "frac": res[head]["force_kernel_tflops"] / peak, "traffic": traffic,
                              "traffic_source": traffic_src,
-                             "kernel": "forceKernel%s (force pair sums)" % ("Mixed" if head == "mixed" else ""),
+                             "kernel": ("forceKernelMixed%s (force pair sums)"
+                                        % ("Rounds" if lib.stat("force_rounds") else "")) if head == "mixed"
+                                       else "forceKernel (force pair sums)",
                              "kernel_avg_ms": res[head]["force_kernel_ms"],
                              "flops_convention": "12 per candidate + 120 per accepted pair"}})
     for k in res:
