@@ -1,7 +1,7 @@
 /*
  * hilbert.hpp -- 63-bit 3-D Hilbert keys (21 octal digits) and their decoding.
  *
- * The curve is the one Cornerstone's hilbert3D produces: levels
+ * The curve is the one Cornerstone's hilbert3D produces, restated from its published algorithm: levels
  * 20..0, digit mortonToHilbert[octant] with table {0,1,3,2,7,6,4,5}, then the
  * coordinate frame is complemented and rotated.  Here the running frame is held as an
  * axis permutation plus a complement bit per axis, packed into one small integer state
@@ -116,7 +116,7 @@ __device__ inline void hilbertDecode(uint64_t key, unsigned& ix, unsigned& iy, u
     }
 }
 
-// hilbert3D quantisation: ix = int(floor(x*mx) - xmin*mx), mx = 2^21/lx,
+// Cornerstone's hilbert3D quantisation: ix = int(floor(x*mx) - xmin*mx), mx = 2^21/lx,
 // clamped to [0, 2^21-1].
 __device__ inline unsigned quantise(double x, double lo, double il)
 {
@@ -129,7 +129,7 @@ __device__ inline unsigned quantise(double x, double lo, double il)
 }
 
 // Node (start key, level) -> FP64 centre and half-size (hilbertIBox + centerAndSize,
-// include/tree.cuh:110-125, SURVEY A.6).
+// include/tree.cuh:110-125).
 __device__ inline void nodeCentreSize(uint64_t start, int level, const DBox& b, double c[3], double s[3])
 {
 #pragma clang fp contract(off)

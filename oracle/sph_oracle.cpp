@@ -22,6 +22,9 @@
  *   gradients             exact-linear velocity field returns its matrix; xi(0) = 1
  *   force                 momentum conservation, energy conservation, zero conductivity
  *                         for uniform u
+ *   periodic boxes        minimum-image superset of the gather and symmetric lists (brute
+ *                         force), wrap invariance of h, rho, gradients and forces
+ *   carried steps         11 solves with h and accelerations carried, GPU against oracle
  *
  * Built with -ffp-contract=off so that every expression is evaluated as written; the
  * HIP kernels evaluate the tree-geometry tests with contraction off as well, which is
@@ -152,8 +155,8 @@ inline Box openBox(int n, const double* x, const double* y, const double* z)
 }
 
 // ---------------------------------------------------------------------------------------
-// Hilbert keys.  Cornerstone iHilbert / hilbert3D restated from their published algorithm
-// (SURVEY appendix A.2): walk levels 20..0, append mortonToHilbert[octant], complement the
+// Hilbert keys.  Cornerstone iHilbert / hilbert3D restated from their published algorithm:
+// walk levels 20..0, append mortonToHilbert[octant], complement the
 // remaining coordinate bits, then rotate (zi) or swap x/z (!yi).  Validity of the curve
 // (bijection, face adjacency, prefix nesting) is pinned by tests; the exact Cornerstone bit
 // pattern is parity unpinned (Cornerstone source absent).
@@ -225,7 +228,7 @@ inline void decodeHilbert(uint64_t key, unsigned out[3])
 }
 
 // Node geometry: prefix (start key, level) -> integer box (hilbertIBox) -> FP64 centre and
-// half-size (centerAndSize, SURVEY A.6): centre = lo + (imax + imin) * (0.5 * u * l),
+// half-size (centerAndSize): centre = lo + (imax + imin) * (0.5 * u * l),
 // size = (imax - imin) * (0.5 * u * l), u = 2^-21.  include/tree.cuh:110-125.
 inline void nodeGeometry(uint64_t start, int level, const Box& b, double c[3], double s[3])
 {
@@ -242,8 +245,9 @@ inline void nodeGeometry(uint64_t start, int level, const Box& b, double c[3], d
     }
 }
 
-// Cornerstone minDistance (box-box), SURVEY A.7: d = |cb - ca| - sa - sb; d = (d + |d|)/2.
-// Periodic: nearest image of cb - ca first (applyPbc, rint); parity unpinned.
+// Cornerstone minDistance (box-box): d = |cb - ca| - sa - sb; d = (d + |d|)/2.
+// Periodic: nearest image of cb - ca first (applyPbc, rint); pinned by the minimum-image
+// superset and wrap-invariance tests (tests/test_oracle_periodic.py).
 inline double minDist2(const double ac[3], const double as[3], const double bc[3],
                        const double bs[3], const Box& b)
 {
@@ -321,7 +325,7 @@ int nLeaves(const State& s) { return (int)s.leafLevel.size(); }
 
 // Canonical leaf tree: Cornerstone updateOctreeGpu iterated to convergence
 // (src/tree.cu:121-136) has the unique fixed point "split iff count > bucket and level < 21"
-// over the live keys (SURVEY A.4).  Built here top-down by recursion.
+// over the live keys.  Built here top-down by recursion.
 static int buildNode(State& s, uint64_t start, int level, unsigned a, unsigned b)
 {
     const int id = (int)s.nodes.size();

@@ -1,4 +1,4 @@
-"""Oracle pin: 63-bit Hilbert keys (Cornerstone hilbert3D, restated).
+"""Oracle pin: 63-bit Hilbert keys (Cornerstone hilbert3D, restated from its published algorithm).
 
 The Cornerstone bit pattern itself cannot be compared (its source is not available), so the
 keys are pinned by what any valid 3-D Hilbert curve of this construction must satisfy:
