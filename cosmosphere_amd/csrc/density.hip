@@ -83,8 +83,12 @@ __device__ inline double xiLimiter(const double g[9])
 // once to FP32; r^2, q, the root, the kernel brackets (kernVD) and the 25 gradient sums are FP32;
 // the W and q D brackets of rho and d(rho)/d(h) are summed in FP64, with the FP32 rounding of
 // 1/h^2 taken out per particle as in the NR pass.  The 3x3 solve and the limiter are FP64.
+//
+// The sweep waits on its gathers more than on arithmetic: the open-box mixed instance runs five
+// wavefronts per SIMD (96 VGPRs, 24 spilled), 2% faster on the 16M-particle mixture than four
+// without spills; the others keep four.
 template<bool PER, bool MIXED>
-__global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(4))) void gradientKernel(GradArgs A)
+__global__ __launch_bounds__(kDensThreads) __attribute__((amdgpu_waves_per_eu(MIXED && !PER ? 5 : 4))) void gradientKernel(GradArgs A)
 {
     using G = std::conditional_t<MIXED, float, double>;   // the gradient sums
     __shared__ int    segStart[kDensWaves][kWave];
