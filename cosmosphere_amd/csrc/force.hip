@@ -295,6 +295,8 @@ __global__ __launch_bounds__(kForceThreads) void forceKernel(ForceArgs A)
 // the square roots and reciprocals, the signal speeds, the viscosity and both kernel gradients
 // are FP32, from FP32 copies of the per-particle factors; each pair's terms go into FP32
 // partials that are added into FP64 accumulators once per tile (at most 64 pairs per lane).
+// Lanes keep their own particle's slots (force_rounds=0); forceKernelMixedRounds, the default,
+// spreads each tile's accepted pairs evenly over the lanes.
 template<bool PER, bool DISC, bool FULL>
 __global__ __launch_bounds__(kForceThreads) __attribute__((amdgpu_waves_per_eu(4))) void forceKernelMixed(ForceArgs A)
 {
@@ -565,7 +567,7 @@ __device__ inline int nthSetBit(uint64_t x, int k)
     return pos;
 }
 
-// Mixed form with shared slots (force_rounds=1): the same lists, culling, tile and pair terms as
+// Mixed form with shared slots (force_rounds=1, the default): the same lists, culling, tile and pair terms as
 // forceKernelMixed, with the accepted pairs of a tile spread evenly over the wavefront.  Lane p
 // owns particle p of the chunk (up to 64 at a time).  For each tile:
 //   1. the support test of each particle runs over the 64 slots at once (its position read as
