@@ -163,11 +163,14 @@ struct ListArgs
     int             nNodes;   // walk cap: a tree walk visits each node at most once
     const PassCtl*  ctl;      // device-driven pass: ctl->nAL leaves, nothing when ctl->skip
     const int*      nodeLeaves;   // long walks: leaves under each internal node
-    int*            heavyN;       // ... lists handed to the block walk (count), nullptr: none
+    int*            heavyN;       // ... [0] lists handed to the block walk (count), nullptr: none;
+                                  //     [1] tokens taken from tokPool
     int*            heavyList;    // ... their positions
     uint8_t*        heavyFlag;    // ... per position: 1 if the block walk owns the list
     int             heavySteps;   // ... hand-over threshold per 32768 lists (kHeavySteps)
     int             expandLeaves; // ... expansion threshold (kExpandLeaves)
+    int2*           tokPool;      // ... the count pass's tokens and their offsets in the list, for the fill
+    uint2*          tokAt;        // ... per hand-over position: (first token in tokPool or ~0u, tokens)
     unsigned long long listMean;  // entries per leaf of the last build of this kind (0: unknown; ctl: its own)
     int             heavyFactor;  // hand-over after heavyFactor x listMean / 4 steps (0: off)
 };
@@ -371,7 +374,37 @@ __device__ inline uint32_t blockScan(uint32_t* __restrict__ v, int n, uint32_t* 
     return total;
 }
 
-// The lists that count walks handed over (heavyList[0, *heavyN)), one block per list.
+// Fill pass of a block walk: the list's tokens at their offsets (a leaf, or out(C) of node C by a
+// group walk), after the list's start jOffset[a].
+template<bool PER, class R>
+__device__ inline void fillTokens(const ListArgs& A, const R& inRange, int a, const int* __restrict__ tk,
+                                  const uint32_t* __restrict__ toff, int nTok, int* __restrict__ stk)
+{
+    const int g = threadIdx.x >> 3, o = threadIdx.x & 7;
+    const uint32_t base = A.jOffset[a], limit = A.jOffset[a + 1];
+    for (int t = g; t < nTok; t += kGroups)
+    {
+        const int v = tk[t];
+        const uint32_t at = base + toff[t];
+        if (v >= 0)
+        {
+            if (o == 0)
+            {
+                if (at < limit) A.jList[at] = v;
+                else            atomicAdd(&A.ovf[0], 1);
+            }
+        }
+        else
+        {
+            bool capped;
+            (void)groupWalk<false>(A, inRange, ~v, stk, at, limit, A.nNodes, capped);
+        }
+    }
+}
+
+// The lists that count walks handed over (heavyList[0, *heavyN)), one block per list.  The count
+// pass keeps each list's tokens and their offsets (tokPool, while it has room), so the fill pass
+// walks each token once more instead of expanding and counting the list again.
 template<bool SYM, bool COUNT, bool PER>
 __global__ __launch_bounds__(kListThreads) void jlistHeavyKernel(ListArgs A)
 {
@@ -386,14 +419,34 @@ __global__ __launch_bounds__(kListThreads) void jlistHeavyKernel(ListArgs A)
     const int g = threadIdx.x >> 3, o = threadIdx.x & 7;
     const int shift = (laneId() >> 3) << 3;
     const uint32_t below = (1u << o) - 1u;
+    __shared__ uint32_t keepAt;
     const int nH = *A.heavyN;
     for (int k = blockIdx.x; k < nH; k += gridDim.x)
     {
         const int a = A.heavyList[k];
         const int iLeaf = A.leaves ? A.leaves[a] : a;
         const Reach<SYM, PER> inRange(A, iLeaf);
-        const int root = A.nodeFirst[A.leafSet[iLeaf]];   // internal and accepted: its walk was long
         int cur = 0, nTok = 1, ne = 1;   // ne: out(C) tokens made by the last round (bounds the next)
+        if constexpr (!COUNT)
+        {
+            // the count pass kept this list's tokens and offsets: only the fill walks remain
+            const uint2 at = A.tokAt[k];
+            if (at.x != ~0u)
+            {
+                nTok = (int)at.y;
+                for (int t = threadIdx.x; t < nTok; t += kListThreads)
+                {
+                    const int2 v = A.tokPool[at.x + (uint32_t)t];
+                    tok[0][t] = v.x;
+                    toff[t] = (uint32_t)v.y;
+                }
+                __syncthreads();
+                fillTokens<PER>(A, inRange, a, tok[0], toff, nTok, stk[g]);
+                __syncthreads();   // the block's LDS is reused by the next list
+                continue;
+            }
+        }
+        const int root = A.nodeFirst[A.leafSet[iLeaf]];   // internal and accepted: its walk was long
         if (threadIdx.x == 0) tok[0][0] = ~root;
         __syncthreads();
         while (ne > 0 && nTok + 7 * ne <= kTokCap)
@@ -462,29 +515,28 @@ __global__ __launch_bounds__(kListThreads) void jlistHeavyKernel(ListArgs A)
         const uint32_t total = blockScan(toff, nTok, red);
         if constexpr (COUNT)
         {
-            if (threadIdx.x == 0) A.jCount[a] = total;
+            // keep the tokens and their offsets for the fill pass while the pool has room
+            if (threadIdx.x == 0)
+            {
+                A.jCount[a] = total;
+                uint32_t at = ~0u;
+                if (A.tokPool)
+                {
+                    const uint32_t s0 = (uint32_t)atomicAdd(&A.heavyN[1], nTok);
+                    if (s0 + (uint32_t)nTok <= (uint32_t)kListTokPool) at = s0;
+                }
+                keepAt = at;
+                if (A.tokAt) A.tokAt[k] = make_uint2(at, (uint32_t)nTok);
+            }
+            __syncthreads();
+            const uint32_t at = keepAt;
+            if (at != ~0u)
+                for (int t = threadIdx.x; t < nTok; t += kListThreads)
+                    A.tokPool[at + (uint32_t)t] = make_int2(tok[cur][t], (int)toff[t]);
         }
         else
         {
-            const uint32_t base = A.jOffset[a], limit = A.jOffset[a + 1];
-            for (int t = g; t < nTok; t += kGroups)
-            {
-                const int v = tok[cur][t];
-                const uint32_t at = base + toff[t];
-                if (v >= 0)
-                {
-                    if (o == 0)
-                    {
-                        if (at < limit) A.jList[at] = v;
-                        else            atomicAdd(&A.ovf[0], 1);
-                    }
-                }
-                else
-                {
-                    bool capped;
-                    (void)groupWalk<false>(A, inRange, ~v, stk[g], at, limit, A.nNodes, capped);
-                }
-            }
+            fillTokens<PER>(A, inRange, a, tok[cur], toff, nTok, stk[g]);
         }
         __syncthreads();   // the block's LDS is reused by the next list
     }
@@ -557,6 +609,8 @@ static ListArgs listArgs(State& s, const int* leaves, int nLeaves, bool symmetri
     a.heavyN = s.heavyN.get<int>(4);
     a.heavyList = s.heavyList.get<int>((size_t)s.nLeaves + 1);
     a.heavyFlag = s.heavyFlag.get<uint8_t>((size_t)s.nLeaves + 1);
+    a.tokPool = s.tokPool.get<int2>((size_t)kListTokPool);
+    a.tokAt = s.tokAt.get<uint2>((size_t)s.nLeaves + 1);
     a.heavySteps = s.listHeavySteps > 0 ? s.listHeavySteps : kHeavySteps;
     a.expandLeaves = s.listExpandLeaves >= 0 ? s.listExpandLeaves : kExpandLeaves;
     a.heavyFactor = s.listHeavyFactor;
@@ -580,7 +634,7 @@ void launchGatherListsDevice(State& s, const int* leaves, int nCap, uint32_t* jC
     }
     else
     {
-        fillU32(reinterpret_cast<uint32_t*>(a.heavyN), 0u, 1, s.stream);
+        fillU32(reinterpret_cast<uint32_t*>(a.heavyN), 0u, 2, s.stream);
         launchList<false, true>(a, per, s.stream);
     }
 }
@@ -593,7 +647,7 @@ long long buildLists(State& s, const int* leaves, int nLeaves, bool symmetric, b
     ListArgs a = listArgs(s, leaves, nLeaves, symmetric, tight);
     a.jCount = jc;
     const bool per = s.hBox.size() && s.hBox[0].periodic;
-    fillU32(reinterpret_cast<uint32_t*>(a.heavyN), 0u, 1, st);
+    fillU32(reinterpret_cast<uint32_t*>(a.heavyN), 0u, 2, st);
     if (nLeaves > 0)
     {
         if (symmetric) launchList<true, true>(a, per, st);

@@ -330,7 +330,8 @@ std::vector<unsigned long long> planKey(const State& s, const Plan& P, int K)
     add(&P, sizeof(Plan));
     for (const DevBuf* b : {&s.layout, &s.leafNode, &s.leafSet, &s.intFirst, &s.nodeChild, &s.nodeCx, &s.nodeCy,
                             &s.nodeCz, &s.nodeSx, &s.nodeSy, &s.nodeSz, &s.hmaxLeaf, &s.hRemLeaf, &s.box, &s.h,
-                            &s.ovf, &s.scanTmp, &s.nodeLeaves, &s.heavyN, &s.heavyList, &s.heavyFlag})
+                            &s.ovf, &s.scanTmp, &s.nodeLeaves, &s.heavyN, &s.heavyList, &s.heavyFlag, &s.tokPool,
+                            &s.tokAt})
         k.push_back((unsigned long long)(uintptr_t)b->p);
     k.push_back((unsigned long long)s.nSets);
     k.push_back((unsigned long long)s.nNodes);
@@ -455,6 +456,8 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
     s.heavyN.get<int>(4);   // the list walks take these in the captured passes: sized here
     s.heavyList.get<int>((size_t)L + 1);
     s.heavyFlag.get<uint8_t>((size_t)L + 1);
+    s.tokPool.get<int2>((size_t)kListTokPool);
+    s.tokAt.get<uint2>((size_t)L + 1);
     if (s.loopBlocks <= 0 || s.loopBlocksPrecision != s.precision || s.loopBlocksPeriodic != (int)in.periodic)
     {
         s.loopBlocks = densityPersistentBlocks(s.precision, in.periodic);

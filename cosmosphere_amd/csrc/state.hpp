@@ -153,6 +153,7 @@ struct State
     DevBuf itemCount, itemCountAlt, itemOff, itemPos, itemPart;   // NR work items
     DevBuf jCount, jOffset, jList;
     DevBuf heavyList, heavyFlag, heavyN;   // lists walked by a whole block (long walks)
+    DevBuf tokPool, tokAt;                 // ... their tokens, kept by the count pass for the fill
     DevBuf qSub, splitPart, splitDone;     // device-driven loop: heavy items split over wavefronts
     DevBuf sweepFlag, sweepCount, sweepOff, sweepItems;   // gradient / force sweeps: per-particle items of long lists
     int    listLeaves = 0;    // number of leaves the CSR lists cover (all-leaf lists)

@@ -11,6 +11,9 @@ namespace csph
 
 struct PassCtl;
 
+// Block walks of long lists: tokens the count pass keeps for the fill pass (16M, 128 MiB).
+constexpr int kListTokPool = 1 << 24;
+
 // Box, keys, sort, permutation (x, y, z, h and, if withVelAcc, v and a), leaf tree, node
 // geometry, layout and particle -> leaf map for the sets described in s.
 // periodicBox: nullptr (open, fitted boxes) or 6 doubles per set.
