@@ -350,6 +350,11 @@ void densityKernel(DensArgs A)
     __shared__ int    tileHead[kNrWaves][kWave];   // first tile position of each list entry (tileOwner)
     __shared__ int rankLane[kNrWaves][kWave];   // whole-leaf items: k-th particle to evaluate -> lane
     __shared__ double cpw[CULL ? kNrWaves : 1][4][CULL ? kWave : 1];   // ... its x, y, z, (R h)^2 slack
+    // FP64 culling: the batch's box (centre, half-sizes) and largest reach (R hmax)^2 slack, and
+    // the j-particles that pass the box test, queued (kCompactJ)
+    constexpr bool kCompactJ = CULL && !kMixed;
+    __shared__ double cbx[kCompactJ ? kNrWaves : 1][8];
+    __shared__ int    jq[kCompactJ ? kNrWaves : 1][kCompactJ ? TD + kWave : 1];
     // whole-leaf tiles: mixed = fixed-point offsets (x, y, z, pad), one 16-byte LDS read per j;
     // plain = FP64 positions
     // (TD slots and the padding read beyond them: TD + 64)
@@ -467,6 +472,20 @@ void densityKernel(DensArgs A)
                         cpw[w][3][r] = (kRad * hc) * (kRad * hc) * kCullSlack;
                     }
                 }
+                if constexpr (kCompactJ)
+                {
+                    // the batch's bounding box and largest reach: a j-particle farther from the box
+                    // than that has no pair with q < R in the batch
+                    const uint32_t icc = want ? ic : p0;
+                    double c3[3], s3[3], rm;
+                    waveBox(want, A.x[icc], A.y[icc], A.z[icc], want ? kRad * A.h[icc] : 0.0, c3, s3, rm);
+                    if (lane == 0)
+                    {
+                        cbx[w][0] = c3[0]; cbx[w][1] = c3[1]; cbx[w][2] = c3[2];
+                        cbx[w][3] = s3[0]; cbx[w][4] = s3[1]; cbx[w][5] = s3[2];
+                        cbx[w][6] = rm * rm * kCullSlack;
+                    }
+                }
                 waveLdsSync();
                 // with culling, a list entry is staged only if its leaf's tight box lies within the support
                 // of one of the c particles (a dropped leaf holds no pair with q < R for them)
@@ -484,6 +503,23 @@ void densityKernel(DensArgs A)
                         (void)jl;
                         return true;
                     }
+                };
+                // FP64: of a kept entry, a j-particle only if it lies within the batch's largest reach
+                // of its box.  It drops 15% of the 8M Plummer mixture's candidates and 7% of the
+                // FP64 solve's time; the mixed form's pair terms cost too little for the test's
+                // extra round trip per 64 slots to pay (its first passes ran 5% slower).
+                auto keepJ = [&](int j) {
+                    const double p3[3] = {A.x[j], A.y[j], A.z[j]};
+                    double d2 = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                    {
+                        double d = cbx[w][k] - p3[k];
+                        if constexpr (PER) d -= box.l[k] * rint(d * box.il[k]);
+                        const double v = fmax(fabs(d) - cbx[w][3 + k], 0.0);
+                        d2 += v * v;
+                    }
+                    return d2 <= cbx[w][6];
                 };
                 const UnitShapes& shapes = kMixed ? kUnitShapesMixed : kUnitShapesPlain;
                 const int U = ((kMixed || CULL) && A.units) ? (int)shapes.U[c] : 1;
@@ -597,7 +633,12 @@ void densityKernel(DensArgs A)
                                 }
                             }
                         };
-                        forEachTileIf<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, load, pad, consumeUnits);
+                        if constexpr (kCompactJ)
+                            forEachTileCompactTD<TD>(segStart[w], segJ[w], tileHead[w], jq[w], A.layout, A.jList, lo, hi,
+                                                     keep, keepJ, load, pad, consumeUnits);
+                        else
+                            forEachTileIf<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, load,
+                                              pad, consumeUnits);
                         // counters by the state of each lane's particles, read before any update
                         if (u0 < nU)
                         {
@@ -717,7 +758,12 @@ void densityKernel(DensArgs A)
                         }
                         cand += sub == 0 ? (uint32_t)nt : 0u;
                     };
-                    forEachTileIf<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, load, pad, consume);
+                    if constexpr (kCompactJ)
+                        forEachTileCompactTD<TD>(segStart[w], segJ[w], tileHead[w], jq[w], A.layout, A.jList, lo, hi,
+                                                 keep, keepJ, load, pad, consume);
+                    else
+                        forEachTileIf<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, load, pad,
+                                          consume);
                     if (has)
                     {
                         const int slot = isClosureEval<MODE>(st) ? 2 : 0;
