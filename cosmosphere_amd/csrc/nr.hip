@@ -472,6 +472,24 @@ void densityKernel(DensArgs A)
                         cpw[w][3][r] = (kRad * hc) * (kRad * hc) * kCullSlack;
                     }
                 }
+                // mixed culling: the batch's box in fixed-point units and its largest reach (R hmax S
+                // + 2 units: each fixed-point position is within half a unit of the particle's) squared,
+                // with slack for the FP32 gap; a j-particle beyond it has no pair with q < R in the batch
+                int bq0 = 0, bq1 = 0, bq2 = 0, bq3 = 0, bq4 = 0, bq5 = 0;
+                float rq2 = 0.0f;
+                if constexpr (kMixed && CULL)
+                {
+                    const uint32_t icc = want ? ic : p0;
+                    const int xq = toFixed(A.x[icc] - c0), yq = toFixed(A.y[icc] - c1), zq = toFixed(A.z[icc] - c2);
+                    bq0 = waveReduceMin(want ? xq : 0x7fffffff);
+                    bq1 = waveReduceMin(want ? yq : 0x7fffffff);
+                    bq2 = waveReduceMin(want ? zq : 0x7fffffff);
+                    bq3 = waveReduceMax(want ? xq : -0x7fffffff);
+                    bq4 = waveReduceMax(want ? yq : -0x7fffffff);
+                    bq5 = waveReduceMax(want ? zq : -0x7fffffff);
+                    const double reach = waveReduceMax(want ? kRad * A.h[icc] : 0.0) * S + 2.0;
+                    rq2 = (float)(reach * reach * kCullSlack);
+                }
                 if constexpr (kCompactJ)
                 {
                     // the batch's bounding box and largest reach: a j-particle farther from the box
@@ -555,6 +573,21 @@ void densityKernel(DensArgs A)
                     else { tx[w][k] = 1e200; ty[w][k] = 1e200; tz[w][k] = 1e200; }
                 };
                 auto pad = [&](int slot) { padAt(slot < kWave ? pos0 : pos1); };
+                // mixed culling: a slot's fixed-point offsets, and whether they lie within the
+                // batch's reach of its box
+                auto probe = [&](int j, bool& ok) {
+                    double xj = A.x[j] - c0, yj = A.y[j] - c1, zj = A.z[j] - c2;
+                    imageFP64<PER>(xj, yj, zj, box);
+                    const int xq = toFixed(xj), yq = toFixed(yj), zq = toFixed(zj);
+                    const float dx = (float)max(max(bq0 - xq, xq - bq3), 0);
+                    const float dy = (float)max(max(bq1 - yq, yq - bq4), 0);
+                    const float dz = (float)max(max(bq2 - zq, zq - bq5), 0);
+                    ok = dx * dx + dy * dy + dz * dz <= rq2;
+                    return make_int4(xq, yq, zq, 0);
+                };
+                auto put = [&](int k, const int4& v) {
+                    if constexpr (kMixed) tq[w][k] = v;
+                };
                 // the tiles stage slots 0 .. 63; the slots a sweep reads beyond them hold padding
                 // through the batch: mixed, slots below 2 S + 63 <= 127 (the strided sweeps with
                 // their read-ahead); FP64, slots below Sl Tl <= 95 (sweep order, the read-ahead
@@ -636,6 +669,9 @@ void densityKernel(DensArgs A)
                         if constexpr (kCompactJ)
                             forEachTileCompactTD<TD>(segStart[w], segJ[w], tileHead[w], jq[w], A.layout, A.jList, lo, hi,
                                                      keep, keepJ, load, pad, consumeUnits);
+                        else if constexpr (kMixed && CULL)
+                            forEachTileInPlace<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep,
+                                                   probe, put, padAt, consumeUnits);
                         else
                             forEachTileIf<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, load,
                                               pad, consumeUnits);
@@ -761,6 +797,9 @@ void densityKernel(DensArgs A)
                     if constexpr (kCompactJ)
                         forEachTileCompactTD<TD>(segStart[w], segJ[w], tileHead[w], jq[w], A.layout, A.jList, lo, hi,
                                                  keep, keepJ, load, pad, consume);
+                    else if constexpr (kMixed && CULL)
+                        forEachTileInPlace<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, probe,
+                                               put, padAt, consume);
                     else
                         forEachTileIf<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, load, pad,
                                           consume);

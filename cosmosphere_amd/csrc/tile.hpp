@@ -225,6 +225,86 @@ __device__ inline void forEachTileCompactTD(int* __restrict__ segStart, int* __r
     }
 }
 
+// Tiles of at most TD slots (64 or 128) filled in place: the j-particles of the list entries
+// keep(jl) accepts, those probe(j, ok) accepts, in list order.  probe returns the slot's value
+// (computed before the test, which it may use) and put(position, value) writes it at the next
+// free position of the tile.  A tile is consumed once a round of 64 might not fit (more than
+// TD - 64 slots: tiles of TD - 63 .. TD slots, and the last one), its positions past the end
+// padded by pad(position).  No queue beyond the tile itself.
+template<int TD, class Keep, class Probe, class Put, class Pad, class Consume>
+__device__ inline void forEachTileInPlace(int* __restrict__ segStart, int* __restrict__ segJ, int* __restrict__ head,
+                                          const uint32_t* __restrict__ layout, const int* __restrict__ jList,
+                                          uint32_t lo, uint32_t hi, Keep keep, Probe probe, Put put, Pad pad,
+                                          Consume consume)
+{
+    static_assert(TD == kWave || TD == 2 * kWave, "tile depth 64 or 128");
+    const int lane = laneId();
+    int fill = 0;              // slots in the tile (wave-uniform)
+    int total = 0, s0 = 0;     // the staged chunk's slots, the next slot batch (wave-uniform)
+    int start = 0, cnt = 0;    // this lane's entry of the chunk: first slot, slots
+    uint32_t e0 = lo;
+    bool done = false;
+    while (!done)
+    {
+        while (fill <= TD - kWave)
+        {
+            if (s0 >= total)
+            {
+                if (e0 >= hi)
+                {
+                    done = true;
+                    break;
+                }
+                // the next 64 list entries: their slot ranges
+                waveLdsSync();   // segStart and segJ are rewritten
+                const uint32_t e = e0 + lane;
+                int js = 0;
+                cnt = 0;
+                if (e < hi)
+                {
+                    const int jl = jList[e];
+                    if (keep(jl))
+                    {
+                        js = (int)layout[jl];
+                        cnt = (int)layout[jl + 1] - js;
+                    }
+                }
+                const int incl = waveInclusiveScan(cnt);
+                total = lastLane(incl);   // wave-uniform
+                start = incl - cnt;
+                segStart[lane] = start;
+                segJ[lane] = js;
+                waveLdsSync();
+                e0 += kWave;
+                s0 = 0;
+                continue;
+            }
+            const int slot = s0 + lane;
+            const int a = tileOwner(head, s0, start, cnt);
+            const bool in = slot < total;
+            const int j = in ? segJ[a] + slot - segStart[a] : segJ[0];
+            bool ok = false;
+            const auto v = probe(j, ok);
+            ok = ok && in;
+            const uint64_t m = ballot64(ok);
+            if (ok) put(fill + (int)__popcll(m & lanemaskLt()), v);   // < TD
+            fill += __builtin_amdgcn_readfirstlane(__popcll(m));
+            s0 += kWave;
+        }
+        if (fill == 0) continue;   // (done: the loop ends)
+#pragma unroll
+        for (int part = 0; part < TD / kWave; ++part)
+        {
+            const int k = part * kWave + lane;
+            if (k >= fill) pad(k);
+        }
+        waveLdsSync();
+        consume(fill);
+        waveLdsSync();
+        fill = 0;
+    }
+}
+
 constexpr double kCullSlack = 1.0 + 1e-5;
 
 // Bounding box of the points of the lanes with own set (centre c, half-sizes s rounded up so
