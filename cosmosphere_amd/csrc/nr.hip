@@ -354,7 +354,6 @@ void densityKernel(DensArgs A)
     // the j-particles that pass the box test, queued (kCompactJ)
     constexpr bool kCompactJ = CULL && !kMixed;
     __shared__ double cbx[kCompactJ ? kNrWaves : 1][8];
-    __shared__ int    jq[kCompactJ ? kNrWaves : 1][kCompactJ ? TD + kWave : 1];
     // whole-leaf tiles: mixed = fixed-point offsets (x, y, z, pad), one 16-byte LDS read per j;
     // plain = FP64 positions
     // (TD slots and the padding read beyond them: TD + 64)
@@ -526,8 +525,8 @@ void densityKernel(DensArgs A)
                 // of its box.  It drops 15% of the 8M Plummer mixture's candidates and 7% of the
                 // FP64 solve's time; the mixed form's pair terms cost too little for the test's
                 // extra round trip per 64 slots to pay (its first passes ran 5% slower).
-                auto keepJ = [&](int j) {
-                    const double p3[3] = {A.x[j], A.y[j], A.z[j]};
+                auto keepJ = [&](const double3& pj) {
+                    const double p3[3] = {pj.x, pj.y, pj.z};
                     double d2 = 0.0;
 #pragma unroll
                     for (int k = 0; k < 3; ++k)
@@ -588,6 +587,26 @@ void densityKernel(DensArgs A)
                 auto put = [&](int k, const int4& v) {
                     if constexpr (kMixed) tq[w][k] = v;
                 };
+                // FP64 culling: a slot's position, and whether it lies within the batch's reach of
+                // its box; slot k goes to its sweep-order position
+                const uint32_t mSl = (65536u + (uint32_t)Sl - 1u) / (uint32_t)Sl;   // k / Sl = (k mSl) >> 16, k < 1024
+                auto posOf = [&](int k) {
+                    const int q = (int)(((uint32_t)k * mSl) >> 16);
+                    return (k - q * Sl) * Tl + q;
+                };
+                auto probe64 = [&](int j, bool& ok) {
+                    const double3 p = make_double3(A.x[j], A.y[j], A.z[j]);
+                    ok = keepJ(p);
+                    return p;
+                };
+                auto put64 = [&](int k, const double3& p) {
+                    if constexpr (!kMixed)
+                    {
+                        const int q = posOf(k);
+                        tx[w][q] = p.x; ty[w][q] = p.y; tz[w][q] = p.z;
+                    }
+                };
+                auto pad64 = [&](int k) { padAt(posOf(k)); };
                 // the tiles stage slots 0 .. 63; the slots a sweep reads beyond them hold padding
                 // through the batch: mixed, slots below 2 S + 63 <= 127 (the strided sweeps with
                 // their read-ahead); FP64, slots below Sl Tl <= 95 (sweep order, the read-ahead
@@ -667,8 +686,8 @@ void densityKernel(DensArgs A)
                             }
                         };
                         if constexpr (kCompactJ)
-                            forEachTileCompactTD<TD>(segStart[w], segJ[w], tileHead[w], jq[w], A.layout, A.jList, lo, hi,
-                                                     keep, keepJ, load, pad, consumeUnits);
+                            forEachTileInPlace<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep,
+                                                   probe64, put64, pad64, consumeUnits);
                         else if constexpr (kMixed && CULL)
                             forEachTileInPlace<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep,
                                                    probe, put, padAt, consumeUnits);
@@ -795,8 +814,8 @@ void densityKernel(DensArgs A)
                         cand += sub == 0 ? (uint32_t)nt : 0u;
                     };
                     if constexpr (kCompactJ)
-                        forEachTileCompactTD<TD>(segStart[w], segJ[w], tileHead[w], jq[w], A.layout, A.jList, lo, hi,
-                                                 keep, keepJ, load, pad, consume);
+                        forEachTileInPlace<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep,
+                                               probe64, put64, pad64, consume);
                     else if constexpr (kMixed && CULL)
                         forEachTileInPlace<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, probe,
                                                put, padAt, consume);

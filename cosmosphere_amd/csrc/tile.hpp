@@ -149,82 +149,6 @@ __device__ inline void forEachTileCompact(int* __restrict__ segStart, int* __res
     }
 }
 
-// forEachTileCompact with tiles of TD slots (64 or 128; the last one padded by pad(slot)):
-// the j-particles of the list entries keep(jl) accepts, those keepJ(j) accepts, in list order,
-// TD at a time.  queue: TD + 64 ints of LDS per wavefront.  One call site of consume.
-template<int TD, class Keep, class KeepJ, class Load, class Pad, class Consume>
-__device__ inline void forEachTileCompactTD(int* __restrict__ segStart, int* __restrict__ segJ, int* __restrict__ head,
-                                            int* __restrict__ queue, const uint32_t* __restrict__ layout,
-                                            const int* __restrict__ jList, uint32_t lo, uint32_t hi, Keep keep,
-                                            KeepJ keepJ, Load load, Pad pad, Consume consume)
-{
-    static_assert(TD == kWave || TD == 2 * kWave, "tile depth 64 or 128");
-    const int lane = laneId();
-    int fill = 0;              // queued j, wave-uniform
-    int total = 0, s0 = 0;     // the staged chunk's slots, the next slot batch (wave-uniform)
-    int start = 0, cnt = 0;    // this lane's entry of the chunk: first slot, slots
-    uint32_t e0 = lo;
-    for (;;)
-    {
-        // queue j-particles until a tile is full or the list is done
-        while (fill < TD)
-        {
-            if (s0 >= total)
-            {
-                if (e0 >= hi) break;
-                // the next 64 list entries: their slot ranges
-                waveLdsSync();   // segStart and segJ are rewritten
-                const uint32_t e = e0 + lane;
-                int js = 0;
-                cnt = 0;
-                if (e < hi)
-                {
-                    const int jl = jList[e];
-                    if (keep(jl))
-                    {
-                        js = (int)layout[jl];
-                        cnt = (int)layout[jl + 1] - js;
-                    }
-                }
-                const int incl = waveInclusiveScan(cnt);
-                total = lastLane(incl);   // wave-uniform
-                start = incl - cnt;
-                segStart[lane] = start;
-                segJ[lane] = js;
-                waveLdsSync();
-                e0 += kWave;
-                s0 = 0;
-                continue;
-            }
-            const int slot = s0 + lane;
-            const int a = tileOwner(head, s0, start, cnt);
-            int j = -1;
-            if (slot < total) j = segJ[a] + slot - segStart[a];
-            const bool ok = j >= 0 && keepJ(j);
-            const uint64_t m = ballot64(ok);
-            if (ok) queue[fill + __popcll(m & lanemaskLt())] = j;   // < TD + kWave
-            fill += __builtin_amdgcn_readfirstlane(__popcll(m));
-            s0 += kWave;
-        }
-        if (fill == 0) break;
-        waveLdsSync();
-        const int nt = min(fill, TD);
-        const int rest = queue[TD + lane];   // entries TD .. fill - 1 move to the front after the tile
-#pragma unroll
-        for (int part = 0; part < TD / kWave; ++part)
-        {
-            const int k = part * kWave + lane;
-            if (k < nt) load(k, queue[k]);
-            else        pad(k);
-        }
-        waveLdsSync();
-        consume(nt);
-        waveLdsSync();
-        fill -= nt;
-        if (lane < fill) queue[lane] = rest;
-    }
-}
-
 // Tiles of at most TD slots (64 or 128) filled in place: the j-particles of the list entries
 // keep(jl) accepts, those probe(j, ok) accepts, in list order.  probe returns the slot's value
 // (computed before the test, which it may use) and put(position, value) writes it at the next
