@@ -173,6 +173,8 @@ struct ListArgs
     uint2*          tokAt;        // ... per hand-over position: (first token in tokPool or ~0u, tokens)
     unsigned long long listMean;  // entries per leaf of the last build of this kind (0: unknown; ctl: its own)
     int             heavyFactor;  // hand-over after heavyFactor x listMean / 4 steps (0: off)
+    const int*      fullStart;    // the full-walk order (buildTree): per node, its start; nullptr: off
+    const int*      fullOrder;    // ... per position, the leaf
 };
 
 // The accept test of one i-leaf's list (include/tree.cuh:228-252 and the tight test).
@@ -204,6 +206,41 @@ struct Reach
         {
             const double hr = A.hRemLeaf[iLeaf];
             tight2 = (kRad * hr) * (kRad * hr) * (1.0 + 1e-5);
+        }
+    }
+
+    // Whether every node under `node` passes the test: the node's box lies within the reach, with
+    // a margin (1e-10 of the operands' magnitudes) far above the rounding of the descendants' own
+    // tests.  Gather lists: the box inside the i-cell grown by R hmax along each axis and, with
+    // the tight test, its farthest point within the tight reach of the padded i-cell; symmetric
+    // lists: its farthest point within R hmax_i of the i-cell (every node's reach is at least that).
+    __device__ bool inside(const ListArgs& A, int node) const
+    {
+        const double bc[3] = {A.cx[node], A.cy[node], A.cz[node]};
+        const double bs[3] = {A.sx[node], A.sy[node], A.sz[node]};
+        double far2 = 0.0, scale2 = 0.0;
+        bool in = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+        {
+            double dx = bc[a] - ic[a];
+            if constexpr (PER) dx -= box.l[a] * rint(dx * box.il[a]);
+            const double ad = fabs(dx);
+            const double mag = ad + bs[a] + aSize[a] + pad[a];
+            if constexpr (!SYM) in = in && (ad + bs[a] <= aSize[a] - 1e-10 * mag);
+            const double e = fmax(ad + bs[a] - (SYM ? is[a] : is[a] + pad[a]), 0.0);
+            far2 += e * e;
+            scale2 += mag * mag;
+        }
+        if constexpr (SYM)
+        {
+            return far2 <= radHi * radHi - 1e-10 * (radHi * radHi + scale2);
+        }
+        else
+        {
+            if (!in) return false;
+            if (tight2 < 0.0) return true;
+            return far2 <= tight2 - 1e-10 * (tight2 + scale2);
         }
     }
 
@@ -255,8 +292,12 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
         const int  cref = A.nodeChild[c];
         const bool isLeaf = acc && cref < 0;
         const bool isInt  = acc && cref >= 0;
+        // an internal child whose whole subtree passes: the count takes its leaves at once, the
+        // fill copies them from the full-walk order when the walk would pop it
+        const bool full = isInt && A.fullOrder != nullptr && inRange.inside(A, c);
+        const bool walk = isInt && !(COUNT && full);
         const uint32_t mL = (uint32_t)(ballot64(isLeaf) >> shift) & 0xFFu;
-        const uint32_t mI = (uint32_t)(ballot64(isInt) >> shift) & 0xFFu;
+        const uint32_t mI = (uint32_t)(ballot64(walk) >> shift) & 0xFFu;
         if (!COUNT && isLeaf)
         {
             const uint32_t at = base + count + __popc(mL & below);
@@ -264,10 +305,21 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
             else            atomicAdd(&A.ovf[0], 1);
         }
         count += __popc(mL);
+        if constexpr (COUNT)
+        {
+            int nl = full ? A.nodeLeaves[c] : 0;   // summed over the group's 8 lanes
+            nl += __shfl_xor(nl, 1);
+            nl += __shfl_xor(nl, 2);
+            nl += __shfl_xor(nl, 4);
+            count += (uint32_t)nl;
+            // the hand-over budget counts them as walked (about 4 entries a step): a list goes to
+            // the block walk as it would without the shortcut, since its fill walks what is not full
+            steps += nl >> 2;
+        }
         const int push = __popc(mI);
         if (sp + push <= kStack)
         {
-            if (isInt) stk[sp + __popc(mI & below)] = cref;
+            if (walk) stk[sp + __popc(mI & below)] = full ? ~c : cref;   // < 0: a full node to copy
             sp += push;
         }
         else if (o == 0)
@@ -275,15 +327,50 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
             atomicAdd(&A.ovf[1], 1);   // subtree dropped; cannot happen for a depth-21 tree
         }
         waveLdsSync();
-        if (sp == 0) break;
-        first = stk[--sp];
+        bool more = false;
+        while (sp > 0)
+        {
+            const int v = stk[--sp];
+            if (v >= 0)
+            {
+                first = v;
+                more = true;
+                break;
+            }
+            // (fill) a full node: its subtree's leaves, in the walk's order, 64 per round trip
+            const int node = ~v;
+            const int n = A.nodeLeaves[node], f0 = A.fullStart[node];
+            for (int t0 = 0; t0 < n; t0 += 64)
+            {
+                int e[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                {
+                    const int t = t0 + 8 * u + o;
+                    e[u] = t < n ? A.fullOrder[f0 + t] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                {
+                    const int t = t0 + 8 * u + o;
+                    if (t < n)
+                    {
+                        const uint32_t at = base + count + (uint32_t)t;
+                        if (at < limit) A.jList[at] = e[u];
+                        else            atomicAdd(&A.ovf[0], 1);
+                    }
+                }
+            }
+            count += (uint32_t)n;
+        }
         waveLdsSync();
+        if (!more) break;
     }
     return count;
 }
 
 template<bool SYM, bool COUNT, bool PER>
-__global__ __launch_bounds__(kListThreads) void jlistKernel(ListArgs A)
+__global__ __launch_bounds__(kListThreads) __attribute__((amdgpu_waves_per_eu(8))) void jlistKernel(ListArgs A)
 {
     __shared__ int stk[kGroups][kStack];
     const int g = threadIdx.x >> 3, o = threadIdx.x & 7;
@@ -610,6 +697,8 @@ static ListArgs listArgs(State& s, const int* leaves, int nLeaves, bool symmetri
     a.heavyList = s.heavyList.get<int>((size_t)s.nLeaves + 1);
     a.heavyFlag = s.heavyFlag.get<uint8_t>((size_t)s.nLeaves + 1);
     a.tokPool = s.tokPool.get<int2>((size_t)kListTokPool);
+    a.fullStart = s.fullStart.as<int>();
+    a.fullOrder = s.fullStart.p ? s.fullOrder.as<int>() : nullptr;
     a.tokAt = s.tokAt.get<uint2>((size_t)s.nLeaves + 1);
     a.heavySteps = s.listHeavySteps > 0 ? s.listHeavySteps : kHeavySteps;
     a.expandLeaves = s.listExpandLeaves >= 0 ? s.listExpandLeaves : kExpandLeaves;

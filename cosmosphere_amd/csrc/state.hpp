@@ -142,6 +142,7 @@ struct State
     DevBuf nodeHmax;     // force upsweep
     DevBuf intNode;      // node id of each internal node (BFS order)
     DevBuf nodeLeaves;   // int: leaves under each internal node
+    DevBuf fullStart, fullOrder;   // the full-walk order: per node its start, per position the leaf
     DevBuf leafBox;      // FP64 tight boxes of the leaves' particles: 3 centres, 3 half-sizes, nLeaves each
     std::vector<int> hIntLevelFirst;          // level ranges of internal nodes (all sets)
 

@@ -426,6 +426,53 @@ __global__ __launch_bounds__(256) void nodeLeavesKernel(int nInt, const uint64_t
     nodeLeaves[intNode[k]] = lower(end) - lower(start);
 }
 
+// The full-walk order: the list the depth-first walk of a list would produce from node N if
+// every node under N were accepted, out_full(N) = N's leaf children in octant order, then
+// out_full(C) of its internal children C in reverse octant order (the LIFO pops).  Each
+// node's out_full is a contiguous range of one permutation of the leaves: fullStart[N] is
+// where it begins (its length is nodeLeaves[N]), fullOrder[position] the leaf.  Built top
+// down, one launch per level; a set's root starts at the set's first leaf.
+__global__ void fullRootKernel(const int* __restrict__ nodeFirst, const int* __restrict__ leafFirst,
+                               const int* __restrict__ intN, const int* __restrict__ nodeChild, int nSets,
+                               int* __restrict__ fullStart, int* __restrict__ fullOrder)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nSets) return;
+    const int root = nodeFirst[b];
+    fullStart[root] = leafFirst[b];
+    if (intN[b] == 0) fullOrder[leafFirst[b]] = ~nodeChild[root];
+}
+
+__global__ void fullLevelKernel(const int* __restrict__ intLevel, const int* __restrict__ intNode, int nInt, int level,
+                                const int* __restrict__ nodeChild, const int* __restrict__ nodeLeaves,
+                                int* __restrict__ fullStart, int* __restrict__ fullOrder)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nInt || intLevel[k] != level) return;
+    const int node = intNode[k];
+    const int c0 = nodeChild[node];
+    int ref[8];
+    int nLeafChildren = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d)
+    {
+        ref[d] = nodeChild[c0 + d];
+        nLeafChildren += ref[d] < 0 ? 1 : 0;
+    }
+    const int s0 = fullStart[node];
+    int pl = s0, pi = s0 + nLeafChildren;
+#pragma unroll
+    for (int d = 0; d < 8; ++d)
+        if (ref[d] < 0) fullOrder[pl++] = ~ref[d];
+#pragma unroll
+    for (int d = 7; d >= 0; --d)
+        if (ref[d] >= 0)
+        {
+            fullStart[c0 + d] = pi;
+            pi += nodeLeaves[c0 + d];
+        }
+}
+
 // Tight box of each leaf's particles: centre and half-size per axis, the half-size rounded up
 // by 2^-50 of the coordinates' magnitude so that the box contains every particle.  One
 // wavefront per leaf.
@@ -798,6 +845,19 @@ void buildTree(State& s, bool withVelAcc, const double* periodicBox, SortScratch
         nodeLeavesKernel<<<iceil(nInt, 256), 256, 0, st>>>(nInt, intStart, intLevel, intSet, t.intNode, t.leafStart,
                                                             dLeafFirst, dIntN, nodeLeaves);
         LAUNCH_CHECK();
+    }
+    {
+        int* fullStart = s.fullStart.get<int>(M + 1);
+        int* fullOrder = s.fullOrder.get<int>(L + 1);
+        fullRootKernel<<<iceil(B, 64), 64, 0, st>>>(dNodeFirst, dLeafFirst, dIntN, t.nodeChild, B, fullStart, fullOrder);
+        LAUNCH_CHECK();
+        if (nInt > 0)
+            for (int lvl = 0; lvl < kMaxLevel; ++lvl)
+            {
+                fullLevelKernel<<<iceil(nInt, 256), 256, 0, st>>>(s.intLevel.as<int>(), t.intNode, nInt, lvl, t.nodeChild,
+                                                                 nodeLeaves, fullStart, fullOrder);
+                LAUNCH_CHECK();
+            }
     }
     uint32_t* layout = s.layout.get<uint32_t>(L + 2);
     layoutKernel<<<iceil(L + 1, 256), 256, 0, st>>>(keys, t.leafStart, t.leafSet, L, s.aliveFirst.as<int>(),
