@@ -45,6 +45,12 @@ FP64_PEAK = 78.6    # TFLOP/s, MI355X vector FP64
 # Flop convention of the density pair sum (fixed): 9 per candidate pair (3 sub, r^2, scale),
 # 21 more per accepted pair (sqrt, kernel polynomial, accumulation).
 FLOPS_CAND, FLOPS_ACC = 9.0, 21.0
+# A candidate is a j-particle in a leaf of i's list, for each evaluated i: the lists are the
+# reference's, so the count is the algorithm's, whatever the kernel culls before its pair test.
+# The kernel's own candidates give the "executed" figure beside it.
+CONVENTION_NOTE = ("9 per candidate + 21 per accepted pair; candidates = j-particles of the list "
+                   "leaves of each evaluated particle (list_candidates); *_executed: the kernel's "
+                   "own candidates after its culling")
 # Flop convention of the force pair sum (fixed): 12 per candidate pair (separation, r^2, the two
 # support tests), 120 more per accepted pair (unit vector, signal speeds, viscosity, both kernel
 # gradients, the i- and j-side force, heating and div v terms)
@@ -69,7 +75,7 @@ def load_config(name):
     raise SystemExit("unknown config " + name)
 
 
-def run_precision(lib, c, prec, steps, warmup, opts=()):
+def run_precision(lib, c, prec, steps, warmup, opts=(), count_lists=True):
     n = c["n"]
     lib.set_option("precision", prec)
     lib.set_option("max_iter", c.get("max_iter", 100))
@@ -102,7 +108,14 @@ def run_precision(lib, c, prec, steps, warmup, opts=()):
     t1 = time.perf_counter()
     wall = t1 - t0
     alive = lib.stat("n_alive")
-    flops = FLOPS_CAND * acc["candidates"] + FLOPS_ACC * acc["pairs"]
+    if count_lists:
+        lc, lc_pairs = list_candidates(lib, one)
+    else:   # --no-list-count: the kernel's own candidates stand in
+        lc, lc_pairs = acc["candidates"] / steps, acc["pairs"] / steps
+    # the convention's candidates are the lists' (every j-particle in the list leaves of each
+    # evaluated particle); the kernel's own, after its culling, give the executed figure
+    flops = FLOPS_CAND * lc * steps + FLOPS_ACC * acc["pairs"]
+    flops_exec = FLOPS_CAND * acc["candidates"] + FLOPS_ACC * acc["pairs"]
     res = {
         "wall_s": wall,
         "ms_per_step": 1e3 * wall / steps,
@@ -111,10 +124,15 @@ def run_precision(lib, c, prec, steps, warmup, opts=()):
         "pairs_per_s_solve": acc["pairs"] / wall,
         "pairs_per_step": acc["pairs"] / steps,
         "candidates_per_step": acc["candidates"] / steps,
+        "list_candidates_per_step": lc if count_lists else None,
+        # the counting solve's pairs against the timed steps' (equal in FP64; the mixed form's
+        # FP32 partials group differently over the longer lists, so its h may round apart)
+        "list_count_pairs_rel_diff": (lc_pairs - acc["pairs"] / steps) / max(acc["pairs"] / steps, 1.0),
         "iterations": acc["iters"] / steps,
         "kernel_launches_per_step": acc["launches"] / steps,
         "kernel_avg_ms": 1e3 * acc["nrkern"] / max(acc["launches"], 1),
         "kernel_tflops": flops / acc["nrkern"] / 1e12 if acc["nrkern"] > 0 else 0.0,
+        "kernel_tflops_executed": flops_exec / acc["nrkern"] / 1e12 if acc["nrkern"] > 0 else 0.0,
         "phase_ms": {k: 1e3 * acc[k] / steps for k in ("bbox", "keys", "sort", "permute", "tree", "nodes", "jbuild",
                                                           "nrkern", "compact", "closure", "total")},
         "n_leaves": lib.stat("n_leaves"),
@@ -126,6 +144,20 @@ def run_precision(lib, c, prec, steps, warmup, opts=()):
 
 
 HBM_PEAK = 8000.0   # GB/s, MI355X HBM3E
+
+
+def list_candidates(lib, solve):
+    """One more solve, untimed, with the list count on: it builds the reference's (untrimmed) lists
+    and counts their candidates for the NR evaluations; returns them and its accepted pairs.  The
+    pairs equal the timed steps' in FP64; the mixed form's FP32 partials group differently over
+    the longer lists, so its pairs may differ by a convergence flip or two.  The count is a
+    separate kernel in each pass, kept out of the timed solves (--no-list-count skips it)."""
+    lib.set_option("count_list", 1)
+    try:
+        solve()
+    finally:
+        lib.set_option("count_list", 0)
+    return lib.stat("list_candidates"), lib.stat("pairs")
 
 
 def streaming_stages(lib, n, acc, steps):
@@ -353,10 +385,19 @@ def bench_ensemble(args):
         lib.sync()
         wall = time.perf_counter() - t0
         alive = lib.stat("n_alive")
-        flops = FLOPS_CAND * tot["candidates"] + FLOPS_ACC * tot["pairs"]
+
+        def again():
+            lib.restore_inputs()
+            lib.batch_solve(pm, 1e-4, 1.2)
+
+        lc = list_candidates(lib, again)[0] if args.count_lists else tot["candidates"] / args.steps
+        flops = FLOPS_CAND * lc * args.steps + FLOPS_ACC * tot["pairs"]
+        flops_exec = FLOPS_CAND * tot["candidates"] + FLOPS_ACC * tot["pairs"]
         out[name] = {"ms_per_step": 1e3 * wall / args.steps, "h_solves_per_s": alive * args.steps / wall,
                      "pairs_per_s_kernel": tot["pairs"] / max(tot["nrkern"], 1e-12),
                      "kernel_tflops": flops / max(tot["nrkern"], 1e-12) / 1e12,
+                     "kernel_tflops_executed": flops_exec / max(tot["nrkern"], 1e-12) / 1e12,
+                     "list_candidates_per_step": lc, "candidates_per_step": tot["candidates"] / args.steps,
                      "kernel_avg_ms": 1e3 * tot["nrkern"] / max(tot["launches"], 1),
                      "iterations": tot["iters"] / args.steps}
         log("  batched %s: %.1f ms/step, %.3e h-solves/s" % (name, out[name]["ms_per_step"],
@@ -374,7 +415,10 @@ def bench_ensemble(args):
         "batched_mixed_vs_sequential_fp64": m["h_solves_per_s"] / (sum(counts) / seq),
         "autotune": tuned,
         "roofline": {"bound": "valu", "achieved": m["kernel_tflops"], "peak": FP32_PEAK, "unit": "TFLOP/s",
-                     "frac": m["kernel_tflops"] / FP32_PEAK, "traffic": None, "kernel_avg_ms": m["kernel_avg_ms"]},
+                     "frac": m["kernel_tflops"] / FP32_PEAK, "traffic": None, "kernel_avg_ms": m["kernel_avg_ms"],
+                     "flops_convention": CONVENTION_NOTE,
+                     "achieved_executed": m["kernel_tflops_executed"],
+                     "frac_executed": m["kernel_tflops_executed"] / FP32_PEAK},
     }
     if not args.no_cpu:
         log("cpu baseline (oracle on a sample of the sets)")
@@ -557,10 +601,26 @@ def bench_force(args, n_override=None):
             pos[2] += v[2] * dt
             h = hnew
             a = f[:3].copy()
+        # the last step's force pass once more (its lists and inputs are still in place, so its
+        # pairs are the same), outside the step totals, with the list count on
+        if args.count_lists:
+            lib.set_option("count_list", 1)
+            try:
+                lib.force(n, pmass, beta=c["beta"], alphau=c["alphau"], disc_viscosity=0, pdv_heating=1,
+                          shock_heating=1)
+            finally:
+                lib.set_option("count_list", 0)
+            again = {k: lib.stat(k) for k in ("force_kernel", "force_pairs", "force_candidates",
+                                              "force_list_candidates")}
+        else:   # --no-list-count: the last step's own figures, its candidates standing in
+            again = {"force_kernel": rec[-1]["force_kernel"], "force_pairs": rec[-1]["force_pairs"],
+                     "force_candidates": rec[-1]["force_candidates"],
+                     "force_list_candidates": rec[-1]["force_candidates"]}
         tot_d = sum(r["density_s"] for r in rec)
         tot_f = sum(r["force_s"] for r in rec)
         tot_k = sum(r["force_kernel"] for r in rec)
         fl = FORCE_FLOPS_CAND * sum(r["force_candidates"] for r in rec) + FORCE_FLOPS_ACC * sum(r["force_pairs"] for r in rec)
+        fl_list = FORCE_FLOPS_CAND * again["force_list_candidates"] + FORCE_FLOPS_ACC * again["force_pairs"]
         carried = rec[1:] if len(rec) > 1 else rec
         res[name] = {
             "particle_updates_per_s": n * len(rec) / (tot_d + tot_f),
@@ -583,7 +643,12 @@ def bench_force(args, n_override=None):
             "force_ms": 1e3 * tot_f / len(rec),
             "force_kernel_ms": 1e3 * tot_k / len(rec),
             "force_pair_interactions_per_s": sum(r["force_pairs"] for r in rec) / max(tot_k, 1e-12),
-            "force_kernel_tflops": fl / max(tot_k, 1e-12) / 1e12,
+            "force_kernel_tflops": fl_list / max(again["force_kernel"], 1e-12) / 1e12,
+            "force_kernel_tflops_executed": fl / max(tot_k, 1e-12) / 1e12,
+            "force_list_count_pass": {"kernel_ms": 1e3 * again["force_kernel"], "pairs": again["force_pairs"],
+                                      "candidates": again["force_candidates"],
+                                      "list_candidates": again["force_list_candidates"],
+                                      "pairs_as_last_step": again["force_pairs"] == rec[-1]["force_pairs"]},
             "density_pair_interactions_per_s": sum(r["pairs"] for r in rec) / max(sum(r["nrkern"] for r in rec), 1e-12),
         }
     out = {"metric": METRIC + " (density + force)", "unit": "particle density+force updates/s", "n_gpus": 1,
@@ -604,7 +669,13 @@ def bench_force(args, n_override=None):
                                         % ("Rounds" if lib.stat("force_rounds") else "")) if head == "mixed"
                                        else "forceKernel (force pair sums)",
                              "kernel_avg_ms": res[head]["force_kernel_ms"],
-                             "flops_convention": "12 per candidate + 120 per accepted pair"}})
+                             "flops_convention": ("12 per candidate + 120 per accepted pair; candidates = "
+                                                  "j-particles of the list leaves of each particle, over the "
+                                                  "last step's force pass run again with the list count "
+                                                  "(force_list_count_pass); *_executed: the kernel's own "
+                                                  "candidates after its culling, over every step"),
+                             "achieved_executed": res[head]["force_kernel_tflops_executed"],
+                             "frac_executed": res[head]["force_kernel_tflops_executed"] / peak}})
     for k in res:
         out[k] = res[k]
     if "fp64" in res and "mixed" in res:
@@ -645,6 +716,9 @@ def main():
                     help="comma-separated precision forms of a single-set workload to autotune (default: both, so "
                          "mixed_vs_fp64 compares tuned paths)")
     ap.add_argument("--no-parity", action="store_true", help="skip the full-size parity leg of C1-C3")
+    ap.add_argument("--no-list-count", dest="count_lists", action="store_false",
+                    help="skip the untimed solve (C4: batch solve; C5: force pass) that counts the "
+                         "reference lists' candidates; the roofline then counts the kernel's own")
     ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
                     help="library option (cosmo_ext_set_option), e.g. loop=0 for the host-driven NR loop")
     args = ap.parse_args()
@@ -698,7 +772,7 @@ def main():
             args_opts = [(k, tuned[name][k]) for k in Library.TUNE_OPTIONS]
         else:
             args_opts = args.opts
-        res[name] = run_precision(lib, c, prec, args.steps, args.warmup, args_opts)
+        res[name] = run_precision(lib, c, prec, args.steps, args.warmup, args_opts, args.count_lists)
         r = res[name]
         log("  %s: %.1f ms/step, %.3e h-solves/s, %.3e pairs/s (kernel), kernel %.2f TFLOP/s, iters %.1f"
             % (name, r["ms_per_step"], r["h_solves_per_s"], r["pairs_per_s_kernel"], r["kernel_tflops"],
@@ -738,13 +812,18 @@ def main():
                      "traffic_source": traffic_src,
                      "kernel": "densityKernel (%s NR pair sums)" % head_name,
                      "kernel_avg_ms": head["kernel_avg_ms"],
-                     "flops_convention": "9 per candidate + 21 per accepted pair"},
+                     "flops_convention": CONVENTION_NOTE,
+                     "achieved_executed": head["kernel_tflops_executed"],
+                     "frac_executed": head["kernel_tflops_executed"] / peak},
+        "list_candidates_per_step": head["list_candidates_per_step"],
+        "list_count_pairs_rel_diff": head["list_count_pairs_rel_diff"],
     }
     if "fp64" in res and "mixed" in res:
         f = res["fp64"]
         out["fp64"] = {"value": f["h_solves_per_s"], "ms_per_step": f["ms_per_step"],
                        "pair_interactions_per_s": f["pairs_per_s_kernel"], "kernel_tflops": f["kernel_tflops"],
-                       "roofline_frac": f["kernel_tflops"] / FP64_PEAK, "iterations": f["iterations"],
+                       "roofline_frac": f["kernel_tflops"] / FP64_PEAK,
+                       "kernel_tflops_executed": f["kernel_tflops_executed"], "iterations": f["iterations"],
                        "phase_ms": f["phase_ms"]}
         out["mixed_vs_fp64"] = res["mixed"]["h_solves_per_s"] / f["h_solves_per_s"]
         na = int(lib.stat("n_alive"))

@@ -312,6 +312,7 @@ extern "C" void cosmo_ext_set_option(const char* name, double value)
     else if (nm == "grad_sym_lists") s.gradSymLists = (int)value;
     else if (nm == "split_len") s.splitLen = (int)value;
     else if (nm == "sweep_split") s.sweepSplit = std::max(0, (int)value);
+    else if (nm == "count_list") s.countList = (int)value;
     else if (nm == "list_expand_leaves") s.listExpandLeaves = (int)value;
     else if (nm == "resident") s.resident = (int)value;
     else if (nm == "force_rounds") s.forceRounds = (int)value;
@@ -355,12 +356,15 @@ extern "C" double cosmo_ext_get_stat(const char* name)
     if (nm == "launches") return t.kernLaunches;
     if (nm == "closure_pairs") return t.closurePairs;
     if (nm == "closure_candidates") return t.closureCand;
+    if (nm == "list_candidates") return t.listCand;
+    if (nm == "closure_list_candidates") return t.closureListCand;
     if (nm == "force_jbuild") return f.jbuild;
     if (nm == "force_prep") return f.prep;
     if (nm == "force_kernel") return f.kernel;
     if (nm == "force_total") return f.total;
     if (nm == "force_pairs") return f.pairsAccepted;
     if (nm == "force_candidates") return f.pairsCandidate;
+    if (nm == "force_list_candidates") return f.listCand;
     if (nm == "n_alive") return s.totalAlive;
     if (nm == "n_leaves") return s.nLeaves;
     if (nm == "n_internal") return s.nInternal;
@@ -384,6 +388,7 @@ extern "C" double cosmo_ext_get_stat(const char* name)
     if (nm == "tail_leaves") return s.tailLeaves;
     if (nm == "loop_blocks_scale") return s.loopBlockScale;
     if (nm == "sweep_split") return s.sweepSplit;
+    if (nm == "count_list") return s.countList;
     if (nm == "sweep_items") return s.sweepHeavyItems;
     if (nm == "resident") return s.resident;
     if (nm == "force_rounds") return s.forceRounds;

@@ -51,6 +51,57 @@ __global__ void maxHKernel(const double* __restrict__ h, int n, unsigned long lo
     if (laneId() == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));   // m >= 0
 }
 
+// List candidates of a pass, one wavefront per active leaf position (grid-stride): the particle
+// counts of the list's leaves summed over the lanes, times the leaf's evaluated particles.
+__global__ __launch_bounds__(256) void listCandKernel(const PassCtl* __restrict__ ctl, int nFix,
+                                                      const int* __restrict__ leaves,
+                                                      const uint32_t* __restrict__ jOffset,
+                                                      const uint32_t* __restrict__ jCount,
+                                                      const int* __restrict__ jList,
+                                                      const uint32_t* __restrict__ layout,
+                                                      const uint8_t* __restrict__ state,
+                                                      unsigned long long* __restrict__ out)
+{
+    if (ctl && ctl->skip) return;
+    const int n = ctl ? (int)ctl->nAL : nFix;
+    const int lane = laneId();
+    const int wave0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, waves = (gridDim.x * blockDim.x) >> 6;
+    unsigned long long s1 = 0, s2 = 0;
+    for (int a = wave0; a < n; a += waves)
+    {
+        const uint32_t lo = jOffset[a], hi = jCount ? lo + jCount[a] : jOffset[a + 1];
+        unsigned long long sj = 0;
+        for (uint32_t e = lo + lane; e < hi; e += kWave)
+        {
+            const int jl = jList[e];
+            sj += layout[jl + 1] - layout[jl];
+        }
+        sj = waveReduceSum(sj);
+        const int leaf = leaves ? leaves[a] : a;
+        unsigned long long n1 = 0, n2 = 0;
+        for (uint32_t p = layout[leaf] + lane; p < layout[leaf + 1]; p += kWave)
+        {
+            const uint8_t st = state ? state[p] : 1;
+            n1 += st == 1;
+            n2 += st == 2;
+        }
+        s1 += sj * waveReduceSum(n1);
+        s2 += sj * waveReduceSum(n2);
+    }
+    if (lane == 0 && s1) atomicAdd(out, s1);
+    if (lane == 0 && s2) atomicAdd(out + 1, s2);
+}
+
+void launchListCand(const PassCtl* ctl, int n, const int* leaves, const uint32_t* jOffset, const uint32_t* jCount,
+                    const int* jList, const uint32_t* layout, const uint8_t* state, unsigned long long* out,
+                    hipStream_t st)
+{
+    // a fixed grid when the count is on the device; grid-stride beyond it
+    const int blocks = ctl ? 2048 : std::max(1, std::min(2048, iceil(n, 4)));
+    listCandKernel<<<blocks, 256, 0, st>>>(ctl, n, leaves, jOffset, jCount, jList, layout, state, out);
+    LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------------------
 // Gradient sweep: src/density_base.cu:529-714, one wavefront per leaf, all leaves.
 // ---------------------------------------------------------------------------------------

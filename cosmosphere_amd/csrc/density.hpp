@@ -200,6 +200,13 @@ void launchDeadGradients(int first, int n, double* divv, double* xi, double* ddi
 // Per active leaf: flag (has a particle in state != 0) and its work-item count (1 whole-leaf
 // item, or one item per remaining particle when at most `sparse` remain); counters 2-4.
 // With ctl (device-driven pass), nLeaves is the capacity and ctl->nAL the count.
+// List candidates of a pass: for each active leaf position a (n of them, or ctl->nAL), the
+// particle counts of its list's leaves summed (entries [jOffset[a], jOffset[a] + jCount[a]), or
+// to jOffset[a + 1] without jCount), times the leaf's particles in state 1 into out[0] and in
+// state 2 into out[1]; without state, times all its particles into out[0].
+void launchListCand(const PassCtl* ctl, int n, const int* leaves, const uint32_t* jOffset, const uint32_t* jCount,
+                    const int* jList, const uint32_t* layout, const uint8_t* state, unsigned long long* out,
+                    hipStream_t st);
 void launchLeafFlags(const int* leaves, int nLeaves, const uint32_t* layout, const uint8_t* state, int sparse,
                      uint32_t* flag, uint32_t* itemCount, unsigned long long* counters, hipStream_t st,
                      const PassCtl* ctl = nullptr);

@@ -711,7 +711,8 @@ void launchGatherListsDevice(State& s, const int* leaves, int nCap, uint32_t* jC
                              int* jList, const PassCtl* ctl)
 {
     if (nCap <= 0) return;
-    ListArgs a = listArgs(s, leaves, nCap, false, true);
+    // tight lists, except in a solve that counts list candidates: the reference's lists there
+    ListArgs a = listArgs(s, leaves, nCap, false, !s.countList);
     a.ctl = ctl;
     a.jCount = jCount;
     a.jOffset = jOffset;

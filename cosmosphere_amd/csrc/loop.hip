@@ -272,6 +272,7 @@ struct Plan
     int*        jList;
     unsigned long long listCap;
     unsigned long long* counters;
+    unsigned long long* listCand;
     uint8_t*    state;
     int         L;
     int         maxIter, sparse, tailLeaves, flat;
@@ -296,6 +297,7 @@ void enqueuePass(State& s, const Plan& P)
     listCheckKernel<<<1, 1, 0, st>>>(P.ctl, P.jo, P.listCap);
     LAUNCH_CHECK();
     launchGatherListsDevice(s, P.leaves, P.L, P.jc, P.jo, P.jList, P.ctl);
+    if (P.listCand) launchListCand(P.ctl, P.L, P.leaves, P.jo, P.jc, P.jList, layout, P.state, P.listCand, st);
     scanExclusive<uint32_t>(P.cnt, P.itemOff, P.L, s.scanTmp, nullptr, st, P.ctl);
     launchItemFill(P.leaves, P.L, layout, P.state, P.sparse, P.itemOff, P.itemPos, P.itemPart, st, P.ctl);
     itemCountKernel<<<1, 1, 0, st>>>(P.ctl, P.itemOff);
@@ -427,6 +429,7 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
     P.itemPos = in.itemPos;
     P.itemPart = in.itemPart;
     P.state = s.active.as<uint8_t>();
+    P.listCand = in.listCand;
     P.maxIter = s.maxIter;
     P.sparse = in.sparse;
     P.tailLeaves = s.tailLeaves;

@@ -25,6 +25,7 @@ struct DeviceLoopArgs
     unsigned long long nIterating;
     int       sparse;
     bool      periodic;
+    unsigned long long* listCand;   // list candidates of the passes (nullptr: not counted)
     DensArgs  da;
 };
 

@@ -199,6 +199,14 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         nAL = (int)nNext;
     };
 
+    // list candidates (count_list on, MODE 0 solves): accumulated on the device over the passes
+    unsigned long long* listCand = nullptr;
+    if (s.countList && mode == 0)
+    {
+        listCand = s.listCandCtr.get<unsigned long long>(2);
+        fillU32(reinterpret_cast<uint32_t*>(listCand), 0u, 4, st);
+    }
+
     s.passLog.clear();
     launchIota(leaves, L, st);
     compact(na > 0 ? L : 0, zeroCounters(s), nullptr);   // drops the leaves without live particles
@@ -228,6 +236,7 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         dl.da = densArgs(s, leaves, L, pmass, tolh, hfact, nullptr);
         dl.da.hmaxLeaf = s.hRemLeaf.as<double>();
         dl.da.closure = closure && !withGradients;
+        dl.listCand = listCand;
         nIterating = (unsigned long long)deviceLoop(s, dl, t);
         nAL = 0;
         devLoop = true;
@@ -241,7 +250,12 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
         const bool perParticle = sparse > 0 && (s.mapping == kMapFlat || nAL <= s.tailLeaves);
         a0.rec(st);
         hmaxLeaves(s, leaves, nAL, mode == 0 ? state : nullptr);
-        buildLists(s, leaves, nAL, false, mode == 0);   // tight: reach of the particles still to evaluate
+        // tight: reach of the particles still to evaluate (the reference's lists when counting
+        // list candidates)
+        buildLists(s, leaves, nAL, false, mode == 0 && !s.countList);
+        if (listCand)
+            launchListCand(nullptr, nAL, leaves, s.jOffset.as<uint32_t>(), nullptr, s.jList.as<int>(), layout, state,
+                           listCand, st);
         if (perParticle)
         {
             scanExclusive<uint32_t>(cnt, itemOff, nAL, s.scanTmp, nullptr, st);
@@ -284,6 +298,14 @@ Timings solveDensH(State& s, double pmass, int mode, bool withGradients, const d
     }
     (void)devLoop;
     const unsigned long long nActive = na > 0 ? nIterating : 0;
+    if (listCand)
+    {
+        unsigned long long lc[2] = {0, 0};
+        HIP_CHECK(hipMemcpyAsync(lc, listCand, sizeof(lc), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        t.listCand = (double)lc[0];
+        t.closureListCand = (double)lc[1];
+    }
 
     int hovf[2] = {0, 0};
     HIP_CHECK(hipMemcpyAsync(hovf, ovf, sizeof(hovf), hipMemcpyDeviceToHost, st));
@@ -462,6 +484,17 @@ ForceTimings computeForces(State& s, double pmass, double beta, double alphau, b
     ft.total = secs(e0, e3);
     ft.pairsAccepted = (double)c[0];
     ft.pairsCandidate = (double)c[1];
+    if (s.countList && a.nLeaves > 0)
+    {
+        // after the timed span: the candidates the symmetric lists fix, every particle evaluated
+        unsigned long long* lc = s.listCandCtr.get<unsigned long long>(2);
+        fillU32(reinterpret_cast<uint32_t*>(lc), 0u, 4, st);
+        launchListCand(nullptr, a.nLeaves, nullptr, a.jOffset, nullptr, a.jList, a.layout, nullptr, lc, st);
+        unsigned long long h[2] = {0, 0};
+        HIP_CHECK(hipMemcpyAsync(h, lc, sizeof(h), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        ft.listCand = (double)h[0];
+    }
     s.ft = ft;
     return ft;
 }

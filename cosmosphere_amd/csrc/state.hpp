@@ -51,6 +51,10 @@ struct Timings
     double pairsAccepted = 0, pairsCandidate = 0;   // summed over NR iterations
     double kernLaunches = 0;                         // NR pair-kernel launches
     double closurePairs = 0, closureCand = 0;
+    // with count_list on: the j-particles of the list leaves of every evaluated particle (NR and
+    // closure evaluations) over the reference's lists (that solve builds them untrimmed), the
+    // candidate count the algorithm fixes whatever the lists are trimmed to or the kernel culls
+    double listCand = 0, closureListCand = 0;
     double graphReplays = 0, graphCaptures = 0;      // device-driven loop
     double resortMovers = -1;                        // incremental re-sort: particles re-sorted from scratch (-1: full sort)
 };
@@ -72,6 +76,7 @@ struct ForceTimings
 {
     double upsweep = 0, jbuild = 0, prep = 0, kernel = 0, total = 0;
     double pairsAccepted = 0, pairsCandidate = 0;
+    double listCand = 0;   // with count_list on: the j-particles of every particle's list leaves
 };
 
 enum Precision : int { kPlainFP64 = 0, kMixed = 1 };
@@ -163,6 +168,7 @@ struct State
     DevBuf counters;          // u64 device counters (pairs, candidates, unconverged)
     DevBuf pinnedCounters;    // host-visible copy target
     DevBuf ovf;               // int[2] list truncation / stack drop (never expected)
+    DevBuf listCandCtr;       // u64[2] list candidates of NR and closure evaluations (count_list on)
     DevBuf passCtl;           // device-driven loop: control block, pass records, work queue order
     DevBuf passLogDev;
     DevBuf qOrder, qFlag;
@@ -199,6 +205,7 @@ struct State
     int listExpandLeaves = -1;     // verification: block walk expands nodes above this many leaves (-1: default)
     int splitLen = -1;             // device-driven loop: heavy items with longer lists are split (-1: default, 0: off)
     int forceRounds = 1;           // mixed force pass: 1 shares each tile's accepted pairs over the lanes, 0 lanes keep their own slots
+    int countList = 0;             // count the list candidates of each pass (a separate kernel; for reports), over the reference's untrimmed lists
     int sweepSplit = 512;          // gradient / force sweeps: leaves with more list entries go one particle per wavefront (0: off)
     std::vector<PassRecord> passLog;   // one record per NR pass of the last solve
     int      closureOn = -1;   // -1: on for mixed, off for plain

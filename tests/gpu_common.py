@@ -63,7 +63,7 @@ def dataset(name):
 DEFAULT_ITEMS = {"mapping": 0, "sparse": 16, "closure": -1, "loop": 1, "graph": 1, "graph_passes": 4, "lpt": 1,
                  "loop_list_cap": 0, "cull": 1, "units": 1, "list_heavy_steps": 0, "list_expand_leaves": -1,
                  "split_len": -1, "sweep_split": 512, "tile_depth": 64, "force_rounds": 1,
-                 "list_heavy_factor": 8, "grad_sym_lists": 1}
+                 "list_heavy_factor": 8, "grad_sym_lists": 1, "count_list": 0}
 # every list through the block walk, every node expanded while the token buffer allows
 FORCE_BLOCK_WALK = {"list_heavy_steps": 1, "list_expand_leaves": 0}
 

@@ -127,7 +127,9 @@ def test_force_repeat_reuses_lists():
 def test_force_after_hsml_upload_rebuilds_lists():
     """New smoothing lengths uploaded between the solve and the force pass: the force pass builds
     its symmetric lists again from them (bit-exact against the oracle's at the new h) instead of
-    reusing the lists of the solve's gradient sweep."""
+    reusing the lists of the solve's gradient sweep.  With count_list on, the force pass also
+    counts its list candidates (every particle against every particle of its leaf's list leaves),
+    which the oracle's lists fix exactly."""
     c, v, u = setup(n=8000, seed=4)
     n = c["n"]
     lib = Library(c["kernel"])
@@ -136,7 +138,11 @@ def test_force_after_hsml_upload_rebuilds_lists():
     h2 = out["h"] * (1.0 + 0.2 * workloads.uniform(4, 50, np.arange(n)))
     lib.upload(HSML, h2)
     lib.upload_sorted(THERMO, thermo(c, h2, u))
-    lib.force(n, c["pmass"])
+    lib.set_option("count_list", 1)
+    try:
+        lib.force(n, c["pmass"])
+    finally:
+        lib.set_option("count_list", 0)
     o = O.Oracle(quintic=c["quintic"])
     o.set_particles(c["x"], c["y"], c["z"], c["h"], v=v, a=np.zeros((3, n)))
     o.build()
@@ -145,6 +151,15 @@ def test_force_after_hsml_upload_rebuilds_lists():
     o.hmax_all()
     o.build_lists(symmetric=True)
     np.testing.assert_array_equal(lib.array("jList"), o.get("jList"))
+    jo = o.get("jOffset").astype(np.int64)
+    jl = o.get("jList").astype(np.int64)
+    cnt = np.diff(o.get("layout").astype(np.int64))
+    cs = np.concatenate([[0], np.cumsum(cnt[jl])])
+    ref = int((cnt[:len(jo) - 1] * (cs[jo[1:]] - cs[jo[:-1]])).sum())
+    print("force list candidates", lib.stat("force_list_candidates"), ref, "kernel", lib.stat("force_candidates"),
+          flush=True)
+    assert lib.stat("force_list_candidates") == ref
+    assert 0 < lib.stat("force_candidates") <= ref
 
 
 @pytest.mark.parametrize("variant", [dict(), dict(disc=1), dict(shock=0, quintic=False), dict(sweep=4),

@@ -70,6 +70,26 @@ def test_density_plain_fp64(name):
     assert abs(lib.stat("pairs") - sum(o.get("pairsPerIter"))) <= 1e-6 * sum(o.get("pairsPerIter")) + 2
 
 
+@pytest.mark.parametrize("loop", [1, 0])
+@pytest.mark.parametrize("name", ["plummer_cubic", "dead"])
+def test_list_candidates(name, loop):
+    """count_list: the j-particles of every evaluated particle's list leaves, summed over the NR
+    passes, against the oracle's candidates (each active particle against every particle of its
+    list's leaves).  The counting solve builds the reference's lists (not trimmed to the particles
+    still iterating); device-driven (loop 1) and host-driven (loop 0) passes.  The kernel's own
+    candidates (after its culling) are fewer."""
+    c = dataset(name)
+    lib, iters = gpu_run(c, precision=0, gradients=False, loop=loop, count_list=1)
+    o, unconv = oracle_run(c, gradients=False)
+    assert unconv == 0 and iters == o.geti("itersRun")
+    ref = sum(o.get("candPerIter"))
+    got = lib.stat("list_candidates")
+    print("list candidates", name, loop, got, ref, "kernel candidates", lib.stat("candidates"), flush=True)
+    assert abs(got - ref) <= 1e-6 * ref
+    assert lib.stat("closure_list_candidates") == 0   # FP64: no closure evaluations
+    assert 0 < lib.stat("candidates") <= got
+
+
 WALKS = {"group": {}, "block": FORCE_BLOCK_WALK, "launch_rule": {"list_heavy_factor": 0},
          "early_handover": {"list_heavy_factor": 1}}
 
