@@ -175,6 +175,9 @@ struct ListArgs
     int             heavyFactor;  // hand-over after heavyFactor x listMean / 4 steps (0: off)
     const int*      fullStart;    // the full-walk order (buildTree): per node, its start; nullptr: off
     const int*      fullOrder;    // ... per position, the leaf
+    int*            slotTok;      // light lists: the count walk's tokens, kListSlotTok per position
+                                  // (>= 0: a leaf; < 0: ~C, a node whose whole subtree is listed)
+    uint32_t*       slotN;        // ... per position: their number, ~0u if not kept (the fill walks)
 };
 
 // The accept test of one i-leaf's list (include/tree.cuh:228-252 and the tight test).
@@ -270,14 +273,20 @@ struct Reach
 // the current node): the count, and without COUNT the entries at jList[base ...] below limit.
 // Stops after stepCap steps with capped set.  The stack holds each pushed node's first child
 // (read with the accept test), so a step waits for one round of loads, not two.
-template<bool COUNT, class R>
+// STORE (with COUNT): the walk also keeps its tokens in the fill's order, the fill's own walk:
+// each leaf as it is accepted, each whole subtree (~C) when it is popped; the first
+// kListSlotTok go to tok[], their number (more once past the slot, and then not exact) to *nTok.
+// Counts and steps are COUNT's.
+template<bool COUNT, bool STORE = false, class R>
 __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int root, int* __restrict__ stk,
-                                     uint32_t base, uint32_t limit, int stepCap, bool& capped)
+                                     uint32_t base, uint32_t limit, int stepCap, bool& capped,
+                                     int* __restrict__ tok = nullptr, uint32_t* nTok = nullptr)
 {
+    static_assert(COUNT || !STORE, "tokens are kept by count walks");
     const int o = threadIdx.x & 7;
     const int shift = (laneId() >> 3) << 3;
     const uint32_t below = (1u << o) - 1u;
-    uint32_t count = 0;
+    uint32_t count = 0, nt = 0;
     int sp = 0, first = A.nodeChild[root], steps = 0;
     capped = false;
     while (true)
@@ -295,7 +304,8 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
         // an internal child whose whole subtree passes: the count takes its leaves at once, the
         // fill copies them from the full-walk order when the walk would pop it
         const bool full = isInt && A.fullOrder != nullptr && inRange.inside(A, c);
-        const bool walk = isInt && !(COUNT && full);
+        // (STORE: a list past the slot keeps no more tokens, so it counts them like COUNT)
+        const bool walk = isInt && !(COUNT && full && (!STORE || nt > (uint32_t)kListSlotTok));
         const uint32_t mL = (uint32_t)(ballot64(isLeaf) >> shift) & 0xFFu;
         const uint32_t mI = (uint32_t)(ballot64(walk) >> shift) & 0xFFu;
         if (!COUNT && isLeaf)
@@ -303,6 +313,12 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
             const uint32_t at = base + count + __popc(mL & below);
             if (at < limit) A.jList[at] = ~cref;
             else            atomicAdd(&A.ovf[0], 1);
+        }
+        if constexpr (STORE)
+        {
+            const uint32_t t = nt + __popc(mL & below);
+            if (isLeaf && t < kListSlotTok) tok[t] = ~cref;
+            nt += __popc(mL);
         }
         count += __popc(mL);
         if constexpr (COUNT)
@@ -337,6 +353,13 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
                 more = true;
                 break;
             }
+            if constexpr (STORE)
+            {
+                // a full node (counted when accepted): its token
+                if (o == 0 && nt < kListSlotTok) tok[nt] = v;
+                ++nt;
+                continue;
+            }
             // (fill) a full node: its subtree's leaves, in the walk's order, 64 per round trip
             const int node = ~v;
             const int n = A.nodeLeaves[node], f0 = A.fullStart[node];
@@ -366,7 +389,54 @@ __device__ inline uint32_t groupWalk(const ListArgs& A, const R& inRange, int ro
         waveLdsSync();
         if (!more) break;
     }
+    if constexpr (STORE) *nTok = nt;
     return count;
+}
+
+// (fill) A light list from the tokens its count walk kept: leaves as they are, whole subtrees
+// copied from the full-walk order; the same entries in the same order as the fill walk.
+__device__ inline void expandTokens(const ListArgs& A, int a, uint32_t nT)
+{
+    const int o = threadIdx.x & 7;
+    const int shift = (laneId() >> 3) << 3;
+    const int* tk = A.slotTok + (size_t)a * kListSlotTok;
+    const uint32_t limit = A.jOffset[a + 1];
+    uint32_t at = A.jOffset[a];
+    for (uint32_t t0 = 0; t0 < nT; t0 += 8)
+    {
+        const uint32_t t = t0 + o;
+        const bool has = t < nT;
+        const int v = has ? tk[t] : 0;
+        const uint32_t sz = has ? (v >= 0 ? 1u : (uint32_t)A.nodeLeaves[~v]) : 0u;
+        uint32_t incl = sz;   // inclusive scan over the group's 8 lanes
+        for (int d = 1; d < 8; d <<= 1)
+        {
+            const uint32_t y = __shfl_up(incl, d, 8);
+            if (o >= d) incl += y;
+        }
+        const uint32_t off = at + incl - sz;
+        if (has && v >= 0)
+        {
+            if (off < limit) A.jList[off] = v;
+            else             atomicAdd(&A.ovf[0], 1);
+        }
+        uint32_t fm = (uint32_t)(ballot64(has && v < 0) >> shift) & 0xFFu;
+        while (fm)
+        {
+            const int l = __builtin_ctz(fm);
+            fm &= fm - 1;
+            const int node = ~__shfl(v, l, 8);
+            const uint32_t o0 = __shfl(off, l, 8);
+            const int n = A.nodeLeaves[node], f0 = A.fullStart[node];
+            for (int k = o; k < n; k += 8)
+            {
+                const uint32_t p = o0 + (uint32_t)k;
+                if (p < limit) A.jList[p] = A.fullOrder[f0 + k];
+                else           atomicAdd(&A.ovf[0], 1);
+            }
+        }
+        at += __shfl(incl, 7, 8);
+    }
 }
 
 template<bool SYM, bool COUNT, bool PER>
@@ -390,6 +460,16 @@ __global__ __launch_bounds__(kListThreads) __attribute__((amdgpu_waves_per_eu(8)
     for (int a = blockIdx.x * kGroups + g; a < nL; a += gridDim.x * kGroups)
     {
         if (!COUNT && heavy && A.heavyFlag[a]) continue;   // filled by the block walk
+        if constexpr (!COUNT)
+        {
+            const uint32_t nT = A.slotN[a];
+            if (nT != ~0u)
+            {
+                expandTokens(A, a, nT);
+                continue;
+            }
+        }
+        if (COUNT && o == 0) A.slotN[a] = ~0u;   // kept below once the walk ends within the slot
         const int iLeaf = A.leaves ? A.leaves[a] : a;
         const Reach<SYM, PER> inRange(A, iLeaf);
         const uint32_t base = COUNT ? 0u : A.jOffset[a];
@@ -411,8 +491,11 @@ __global__ __launch_bounds__(kListThreads) __attribute__((amdgpu_waves_per_eu(8)
             continue;
         }
         bool capped;
-        const uint32_t count = groupWalk<COUNT>(A, inRange, root, stk[g], base, COUNT ? 0u : A.jOffset[a + 1],
-                                                (COUNT && heavy) ? heavyCap : A.nNodes, capped);
+        uint32_t nt = 0;
+        const uint32_t count =
+            groupWalk<COUNT, COUNT>(A, inRange, root, stk[g], base, COUNT ? 0u : A.jOffset[a + 1],
+                                    (COUNT && heavy) ? heavyCap : A.nNodes, capped,
+                                    COUNT ? A.slotTok + (size_t)a * kListSlotTok : nullptr, &nt);
         if (capped)
         {
             if (COUNT && heavy)
@@ -426,7 +509,11 @@ __global__ __launch_bounds__(kListThreads) __attribute__((amdgpu_waves_per_eu(8)
             }
             if (o == 0) atomicAdd(&A.ovf[1], 1);
         }
-        if (COUNT && o == 0) A.jCount[a] = count;
+        if (COUNT && o == 0)
+        {
+            A.jCount[a] = count;
+            if (nt <= (uint32_t)kListSlotTok) A.slotN[a] = nt;
+        }
     }
 }
 
@@ -700,6 +787,8 @@ static ListArgs listArgs(State& s, const int* leaves, int nLeaves, bool symmetri
     a.fullStart = s.fullStart.as<int>();
     a.fullOrder = s.fullStart.p ? s.fullOrder.as<int>() : nullptr;
     a.tokAt = s.tokAt.get<uint2>((size_t)s.nLeaves + 1);
+    a.slotTok = s.slotTok.get<int>(((size_t)s.nLeaves + 1) * kListSlotTok);
+    a.slotN = s.slotN.get<uint32_t>((size_t)s.nLeaves + 1);
     a.heavySteps = s.listHeavySteps > 0 ? s.listHeavySteps : kHeavySteps;
     a.expandLeaves = s.listExpandLeaves >= 0 ? s.listExpandLeaves : kExpandLeaves;
     a.heavyFactor = s.listHeavyFactor;

@@ -333,7 +333,7 @@ std::vector<unsigned long long> planKey(const State& s, const Plan& P, int K)
     for (const DevBuf* b : {&s.layout, &s.leafNode, &s.leafSet, &s.intFirst, &s.nodeChild, &s.nodeCx, &s.nodeCy,
                             &s.nodeCz, &s.nodeSx, &s.nodeSy, &s.nodeSz, &s.hmaxLeaf, &s.hRemLeaf, &s.box, &s.h,
                             &s.ovf, &s.scanTmp, &s.nodeLeaves, &s.heavyN, &s.heavyList, &s.heavyFlag, &s.tokPool,
-                            &s.tokAt})
+                            &s.tokAt, &s.slotTok, &s.slotN})
         k.push_back((unsigned long long)(uintptr_t)b->p);
     k.push_back((unsigned long long)s.nSets);
     k.push_back((unsigned long long)s.nNodes);
@@ -461,6 +461,8 @@ int deviceLoop(State& s, const DeviceLoopArgs& in, Timings& t)
     s.heavyFlag.get<uint8_t>((size_t)L + 1);
     s.tokPool.get<int2>((size_t)kListTokPool);
     s.tokAt.get<uint2>((size_t)L + 1);
+    s.slotTok.get<int>(((size_t)L + 1) * kListSlotTok);
+    s.slotN.get<uint32_t>((size_t)L + 1);
     if (s.loopBlocks <= 0 || s.loopBlocksPrecision != s.precision || s.loopBlocksPeriodic != (int)in.periodic)
     {
         s.loopBlocks = densityPersistentBlocks(s.precision, in.periodic);

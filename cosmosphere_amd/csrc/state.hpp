@@ -159,6 +159,7 @@ struct State
     DevBuf itemCount, itemCountAlt, itemOff, itemPos, itemPart;   // NR work items
     DevBuf jCount, jOffset, jList;
     DevBuf heavyList, heavyFlag, heavyN;   // lists walked by a whole block (long walks)
+    DevBuf slotTok, slotN;                 // light lists: the count walk's tokens (kListSlotTok per position) and their number
     DevBuf tokPool, tokAt;                 // ... their tokens, kept by the count pass for the fill
     DevBuf qSub, splitPart, splitDone;     // device-driven loop: heavy items split over wavefronts
     DevBuf sweepFlag, sweepCount, sweepOff, sweepItems;   // gradient / force sweeps: per-particle items of long lists

@@ -13,6 +13,7 @@ struct PassCtl;
 
 // Block walks of long lists: tokens the count pass keeps for the fill pass (16M, 128 MiB).
 constexpr int kListTokPool = 1 << 24;
+constexpr int kListSlotTok = 256;  // light lists: tokens the count walk keeps per list for the fill
 
 // Box, keys, sort, permutation (x, y, z, h and, if withVelAcc, v and a), leaf tree, node
 // geometry, layout and particle -> leaf map for the sets described in s.
