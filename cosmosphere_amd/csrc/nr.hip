@@ -574,8 +574,9 @@ void densityKernel(DensArgs A)
                 auto pad = [&](int slot) { padAt(slot < kWave ? pos0 : pos1); };
                 // mixed culling: a slot's fixed-point offsets, and whether they lie within the
                 // batch's reach of its box
-                auto probe = [&](int j, bool& ok) {
-                    double xj = A.x[j] - c0, yj = A.y[j] - c1, zj = A.z[j] - c2;
+                auto fetch = [&](int j) { return make_double3(A.x[j], A.y[j], A.z[j]); };
+                auto finish = [&](const double3& r, bool& ok) {
+                    double xj = r.x - c0, yj = r.y - c1, zj = r.z - c2;
                     imageFP64<PER>(xj, yj, zj, box);
                     const int xq = toFixed(xj), yq = toFixed(yj), zq = toFixed(zj);
                     const float dx = (float)max(max(bq0 - xq, xq - bq3), 0);
@@ -594,8 +595,7 @@ void densityKernel(DensArgs A)
                     const int q = (int)(((uint32_t)k * mSl) >> 16);
                     return (k - q * Sl) * Tl + q;
                 };
-                auto probe64 = [&](int j, bool& ok) {
-                    const double3 p = make_double3(A.x[j], A.y[j], A.z[j]);
+                auto finish64 = [&](const double3& p, bool& ok) {
                     ok = keepJ(p);
                     return p;
                 };
@@ -686,11 +686,11 @@ void densityKernel(DensArgs A)
                             }
                         };
                         if constexpr (kCompactJ)
-                            forEachTileInPlace<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep,
-                                                   probe64, put64, pad64, consumeUnits);
+                            forEachTileInPlaceAhead<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi,
+                                                        keep, fetch, finish64, put64, pad64, consumeUnits);
                         else if constexpr (kMixed && CULL)
-                            forEachTileInPlace<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep,
-                                                   probe, put, padAt, consumeUnits);
+                            forEachTileInPlaceAhead<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi,
+                                                        keep, fetch, finish, put, padAt, consumeUnits);
                         else
                             forEachTileIf<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, load,
                                               pad, consumeUnits);
@@ -814,11 +814,11 @@ void densityKernel(DensArgs A)
                         cand += sub == 0 ? (uint32_t)nt : 0u;
                     };
                     if constexpr (kCompactJ)
-                        forEachTileInPlace<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep,
-                                               probe64, put64, pad64, consume);
+                        forEachTileInPlaceAhead<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi,
+                                                    keep, fetch, finish64, put64, pad64, consume);
                     else if constexpr (kMixed && CULL)
-                        forEachTileInPlace<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, probe,
-                                               put, padAt, consume);
+                        forEachTileInPlaceAhead<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi,
+                                                    keep, fetch, finish, put, padAt, consume);
                     else
                         forEachTileIf<TD>(segStart[w], segJ[w], tileHead[w], A.layout, A.jList, lo, hi, keep, load, pad,
                                           consume);

@@ -229,6 +229,99 @@ __device__ inline void forEachTileInPlace(int* __restrict__ segStart, int* __res
     }
 }
 
+// forEachTileInPlace with the probe in two parts: fetch(j) loads a slot's raw data and
+// finish(raw, ok) turns it into the slot's value and test.  Before a tile is consumed, the next
+// 64 slots of the staged chunk are fetched, so their loads are in flight during the sweep.  The
+// tiles, their slots and their order are forEachTileInPlace's.
+template<int TD, class Keep, class Fetch, class Finish, class Put, class Pad, class Consume>
+__device__ inline void forEachTileInPlaceAhead(int* __restrict__ segStart, int* __restrict__ segJ,
+                                               int* __restrict__ head, const uint32_t* __restrict__ layout,
+                                               const int* __restrict__ jList, uint32_t lo, uint32_t hi, Keep keep,
+                                               Fetch fetch, Finish finish, Put put, Pad pad, Consume consume)
+{
+    static_assert(TD == kWave || TD == 2 * kWave, "tile depth 64 or 128");
+    const int lane = laneId();
+    int fill = 0;              // slots in the tile (wave-uniform)
+    int total = 0, s0 = 0;     // the staged chunk's slots, the next slot batch (wave-uniform)
+    int start = 0, cnt = 0;    // this lane's entry of the chunk: first slot, slots
+    uint32_t e0 = lo;
+    bool done = false;
+    bool ahead = false;        // slots s0 .. s0 + 63 already fetched (wave-uniform)
+    bool inA = false;
+    decltype(fetch(0)) rawA{};
+    // the slot's particle (a valid index for lanes past the chunk) and whether it is one
+    auto slotJ = [&](bool& in) {
+        const int slot = s0 + lane;
+        const int a = tileOwner(head, s0, start, cnt);
+        in = slot < total;
+        return in ? segJ[a] + slot - segStart[a] : segJ[0];
+    };
+    while (!done)
+    {
+        while (fill <= TD - kWave)
+        {
+            if (s0 >= total)
+            {
+                if (e0 >= hi)
+                {
+                    done = true;
+                    break;
+                }
+                // the next 64 list entries: their slot ranges
+                waveLdsSync();   // segStart and segJ are rewritten
+                const uint32_t e = e0 + lane;
+                int js = 0;
+                cnt = 0;
+                if (e < hi)
+                {
+                    const int jl = jList[e];
+                    if (keep(jl))
+                    {
+                        js = (int)layout[jl];
+                        cnt = (int)layout[jl + 1] - js;
+                    }
+                }
+                const int incl = waveInclusiveScan(cnt);
+                total = lastLane(incl);   // wave-uniform
+                start = incl - cnt;
+                segStart[lane] = start;
+                segJ[lane] = js;
+                waveLdsSync();
+                e0 += kWave;
+                s0 = 0;
+                continue;
+            }
+            bool in = inA;
+            auto raw = rawA;
+            if (!ahead) raw = fetch(slotJ(in));
+            ahead = false;
+            bool ok = false;
+            const auto v = finish(raw, ok);
+            ok = ok && in;
+            const uint64_t m = ballot64(ok);
+            if (ok) put(fill + (int)__popcll(m & lanemaskLt()), v);   // < TD
+            fill += __builtin_amdgcn_readfirstlane(__popcll(m));
+            s0 += kWave;
+        }
+        if (fill == 0) continue;   // (done: the loop ends)
+        if (s0 < total)
+        {
+            rawA = fetch(slotJ(inA));
+            ahead = true;
+        }
+#pragma unroll
+        for (int part = 0; part < TD / kWave; ++part)
+        {
+            const int k = part * kWave + lane;
+            if (k >= fill) pad(k);
+        }
+        waveLdsSync();
+        consume(fill);
+        waveLdsSync();
+        fill = 0;
+    }
+}
+
 constexpr double kCullSlack = 1.0 + 1e-5;
 
 // Bounding box of the points of the lanes with own set (centre c, half-sizes s rounded up so
